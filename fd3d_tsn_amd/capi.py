@@ -1,0 +1,273 @@
+"""ctypes binding of include/fd3d_tsn.h (lib/libfd3d_tsn.so).
+
+There is no CPU fallback: if the HIP library is missing or no GPU is
+visible, :func:`load` raises.  The library is built in-tree by
+``__graft_entry__.build()`` (``make -C fd3d_tsn_amd/csrc``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import cases as C
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfd3d_tsn.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "fd3d_tsn.h")
+
+FD3D_SW, FD3D_FVW = 0, 1
+NOUT = 5
+OUT_NAMES = ("sliprateX", "sliprateZ", "shearstressX", "shearstressZ", "psi")
+FIELDS = ("u", "v", "w", "xx", "yy", "zz", "xy", "yz", "xz")
+FINAL_NAMES = ("ruptime", "rise", "sliptime", "slipX", "slipZ", "stressdropX", "stressdropZ")
+ERRORS = {-1: "FD3D_E_INVALID", -2: "FD3D_E_CFL", -3: "FD3D_E_NOMEM", -4: "FD3D_E_HIP",
+          -5: "FD3D_E_STATE", -6: "FD3D_E_NEWTON", -7: "FD3D_E_NONFINITE"}
+
+
+class FD3DError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("nxt", ctypes.c_int), ("nyt", ctypes.c_int), ("nzt", ctypes.c_int),
+                ("nabc", ctypes.c_int), ("fspace", ctypes.c_int), ("friction", ctypes.c_int),
+                ("nucleation", ctypes.c_int), ("nstations", ctypes.c_int),
+                ("stations", ctypes.POINTER(ctypes.c_int)),
+                ("dh", ctypes.c_float), ("dt", ctypes.c_float), ("damp_s", ctypes.c_float),
+                ("pml_vp", ctypes.c_float), ("pml_fact", ctypes.c_float),
+                ("Sn", ctypes.c_double), ("uini", ctypes.c_double), ("wini", ctypes.c_double),
+                ("f0", ctypes.c_double), ("v0", ctypes.c_double), ("fw", ctypes.c_double),
+                ("hx0", ctypes.c_double), ("hz0", ctypes.c_double), ("perturb", ctypes.c_double),
+                ("RR2", ctypes.c_double), ("TT2", ctypes.c_double), ("strinixI", ctypes.c_double),
+                ("newton_max", ctypes.c_int), ("ring_steps", ctypes.c_int)]
+
+
+_lib = None
+_FP = ctypes.POINTER(ctypes.c_float)
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+def exported_symbols() -> Tuple[str, ...]:
+    """Function names declared in include/fd3d_tsn.h."""
+    import re
+    text = open(HEADER).read()
+    pat = r"^(?:int|void|size_t|double|const\s+char\s*\*)\s*\**\s*(fd3d_[a-z_0-9]+)\s*\("
+    return tuple(sorted(set(re.findall(pat, text, flags=re.M))))
+
+
+def load(path: Optional[str] = None):
+    """Load the HIP library; raises if it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"HIP library {p} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(p)
+    H = ctypes.c_void_p
+    L.fd3d_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(H)]
+    L.fd3d_footprint.restype = ctypes.c_size_t
+    L.fd3d_footprint.argtypes = [ctypes.POINTER(Config)]
+    L.fd3d_upload_medium.argtypes = [H, _FP, _FP, _FP]
+    L.fd3d_upload_fault_sw.argtypes = [H] + [_FP] * 6
+    L.fd3d_upload_fault_fvw.argtypes = [H] + [_DP] * 11
+    L.fd3d_init_state.argtypes = [H]
+    L.fd3d_step.argtypes = [H, ctypes.c_int, _FP, _FP]
+    L.fd3d_sync.argtypes = [H]
+    L.fd3d_steps_done.argtypes = [H]
+    L.fd3d_download_final.argtypes = [H, _FP]
+    L.fd3d_download_field.argtypes = [H, ctypes.c_int, _FP]
+    L.fd3d_upload_field.argtypes = [H, ctypes.c_int, _FP]
+    L.fd3d_set_profiling.argtypes = [H, ctypes.c_int]
+    L.fd3d_kernel_times.argtypes = [H, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _DP,
+                                    ctypes.POINTER(ctypes.c_long)]
+    L.fd3d_step_bytes.restype = ctypes.c_double
+    L.fd3d_step_bytes.argtypes = [H, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
+    L.fd3d_last_error.restype = ctypes.c_char_p
+    L.fd3d_last_error.argtypes = [H]
+    L.fd3d_destroy.argtypes = [H]
+    if path is None:
+        _lib = L
+    return L
+
+
+def _f32(a) -> np.ndarray:
+    return np.asfortranarray(np.asarray(a, dtype=np.float32))
+
+
+def _f64(a) -> np.ndarray:
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(_FP)
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+def make_config(case: C.Case, fvw_scalars: Optional[Dict[str, float]] = None, newton_max: int = 0,
+                ring_steps: int = 0) -> Tuple[Config, np.ndarray]:
+    sta = np.asarray(case.stations, dtype=np.int32).reshape(-1) if case.stations else np.zeros(3, np.int32)
+    cfg = Config()
+    cfg.nxt, cfg.nyt, cfg.nzt, cfg.nabc = case.nxt, case.nyt, case.nzt, case.nabc
+    cfg.fspace = int(case.fspace)
+    fvw = case.friction == "fvw"
+    cfg.friction = FD3D_FVW if fvw else FD3D_SW
+    cfg.nucleation = int(fvw)
+    cfg.nstations = len(case.stations)
+    cfg.stations = sta.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    cfg.dh, cfg.dt, cfg.damp_s = case.dh, case.dt, case.damp_s
+    cfg.pml_vp, cfg.pml_fact = case.pml_vp, case.pml_fact
+    if fvw and fvw_scalars:
+        for name in C.FVW_SCALARS:
+            setattr(cfg, name, float(fvw_scalars[name]))
+    cfg.newton_max = newton_max
+    cfg.ring_steps = ring_steps
+    return cfg, sta
+
+
+def footprint(case: C.Case) -> int:
+    cfg, _sta = make_config(case)
+    return int(load().fd3d_footprint(ctypes.byref(cfg)))
+
+
+class Solver:
+    """One rupture problem resident on the GPU (the replacement of fd3d()).
+
+    Mirrors the reference's flow: the caller fills the module arrays
+    (medium and fault) as fd3d_init()/forwardspecial*() would, then the
+    time loop runs on the device; per-step fault slabs and station samples
+    come back as numpy arrays shaped like the reference's .res streams.
+    """
+
+    def __init__(self, case: C.Case, fault: Optional[Dict] = None, fvw_scalars=None, medium=None,
+                 newton_max: int = 0, ring_steps: int = 0, init: bool = True):
+        self.L = load()
+        self.case = case
+        fvw = case.friction == "fvw"
+        if fvw and fault is None:
+            fault, fvw_scalars = C.fault_fvw(case)
+        self.cfg, self._sta = make_config(case, fvw_scalars, newton_max, ring_steps)
+        self.h = ctypes.c_void_p()
+        rc = self.L.fd3d_create(ctypes.byref(self.cfg), ctypes.byref(self.h))
+        if rc != 0:
+            self.h = None
+            raise FD3DError(rc, "fd3d_create failed")
+        if medium is None:
+            medium = C.material(case, np.float32)
+        lam, mu, d = (_f32(x) for x in medium)
+        self._chk(self.L.fd3d_upload_medium(self.h, _fp(lam), _fp(mu), _fp(d)))
+        if fvw:
+            arrs = [_f64(fault[n]) for n in C.FVW_FIELDS]
+            self._chk(self.L.fd3d_upload_fault_fvw(self.h, *[_dp(a) for a in arrs]))
+        else:
+            if fault is None:
+                fault = C.fault_sw(case, np.float32)
+            arrs = [_f32(fault[n]) for n in C.SW_FIELDS]
+            self._chk(self.L.fd3d_upload_fault_sw(self.h, *[_fp(a) for a in arrs]))
+        if init:
+            self.init()
+
+    def _chk(self, rc: int):
+        if rc != 0:
+            msg = self.L.fd3d_last_error(self.h).decode() if self.h else ""
+            raise FD3DError(rc, msg)
+
+    def init(self):
+        self._chk(self.L.fd3d_init_state(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fd3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------------ run
+    def step(self, nsteps: int, outputs: bool = True) -> Dict[str, np.ndarray]:
+        """Advance nsteps; returns per-step slabs (nsteps, nxtT, nztT) and stations."""
+        case = self.case
+        nxT, nzT = case.fault_shape
+        nsta = len(case.stations)
+        res: Dict[str, np.ndarray] = {}
+        if not outputs:
+            self._chk(self.L.fd3d_step(self.h, nsteps, None, None))
+            return res
+        out = np.empty((nsteps, NOUT, nzT, nxT), dtype=np.float32)
+        seis = np.empty((nsteps, max(nsta, 1), 3), dtype=np.float32)
+        self._chk(self.L.fd3d_step(self.h, nsteps, _fp(out), _fp(seis) if nsta else None))
+        for o, name in enumerate(OUT_NAMES):
+            res[name] = out[:, o].transpose(0, 2, 1)
+        res["seis"] = seis[:, :nsta, :]
+        return res
+
+    def sync(self):
+        self._chk(self.L.fd3d_sync(self.h))
+
+    def final(self) -> Dict[str, np.ndarray]:
+        c = self.case
+        buf = np.empty(7 * c.nxt * c.nzt, dtype=np.float32)
+        self._chk(self.L.fd3d_download_final(self.h, _fp(buf)))
+        buf = buf.reshape(7, c.nzt, c.nxt).transpose(0, 2, 1)
+        return {n: buf[q].copy() for q, n in enumerate(FINAL_NAMES)}
+
+    def field(self, name: str) -> np.ndarray:
+        c = self.case
+        buf = np.empty(c.cells, dtype=np.float32)
+        self._chk(self.L.fd3d_download_field(self.h, FIELDS.index(name), _fp(buf)))
+        return buf.reshape((c.nxt, c.nyt, c.nzt), order="F")
+
+    def set_field(self, name: str, arr: np.ndarray):
+        a = _f32(arr)
+        self._chk(self.L.fd3d_upload_field(self.h, FIELDS.index(name), _fp(a)))
+
+    # ------------------------------------------------------------ profiling
+    def set_profiling(self, on: bool):
+        self._chk(self.L.fd3d_set_profiling(self.h, int(on)))
+
+    def kernel_times(self) -> Dict[str, Tuple[float, int]]:
+        n = 32
+        names = (ctypes.c_char_p * n)()
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_long * n)()
+        k = self.L.fd3d_kernel_times(self.h, n, names, ms, cnt)
+        if k < 0:
+            self._chk(k)
+        return {names[q].decode(): (ms[q], cnt[q]) for q in range(k)}
+
+    def step_bytes(self) -> Tuple[float, int, int]:
+        b = ctypes.c_long()
+        p = ctypes.c_long()
+        v = self.L.fd3d_step_bytes(self.h, ctypes.byref(b), ctypes.byref(p))
+        return v, b.value, p.value
+
+
+def gpu_available() -> bool:
+    """True when a HIP device is visible (checked without initialising torch)."""
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        return False
+    n = ctypes.c_int(0)
+    try:
+        rc = hip.hipGetDeviceCount(ctypes.byref(n))
+    except Exception:
+        return False
+    return rc == 0 and n.value > 0

@@ -323,6 +323,6 @@ def tiny(friction: str = "sw", fspace: bool = True, nabc: int = 6, n=(40, 16, 20
                 fault_len=0.6 * nxT * dh, fault_wid=0.6 * nzT * dh,
                 nuc_size=4 * dh, t0_corr=3 * dh)
     if friction == "fvw":
-        base.update(dt=0.004, nuc_radius=5 * dh, trans=3 * dh, nuc_time=0.05)
+        base.update(dt=0.004, nuc_radius=5 * dh, trans=3 * dh, nuc_time=0.051)
     base.update(kw)
     return Case(**base)

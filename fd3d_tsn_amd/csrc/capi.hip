@@ -1,0 +1,642 @@
+// Host side of the C ABI declared in include/fd3d_tsn.h.
+//
+// One handle = one rupture problem resident in HBM + one HIP stream.
+// Responsibilities: device allocation (fd3d_create), uploads with the
+// padded layout, the fd3d() prologue (zeroing, PML profiles, fault
+// parameter staggering), the per-step kernel sequence, the device output
+// ring and its transfer to the caller, per-kernel event timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fd3d_tsn.h"
+#include "fd3d_dev.h"
+
+using namespace fd3d;
+
+struct fd3d_handle {
+  fd3d_config cfg{};
+  std::vector<int> sta;
+  Problem p{};
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  bool medium_ok = false, fault_ok = false, inited = false;
+  int steps = 0;
+  std::string err;
+  // host copies of the fault inputs (staggering is done on the host)
+  std::vector<float> h_striniX, h_striniZ, h_peak, h_dyn, h_Dc, h_coh;
+  std::vector<double> h_a, h_b, h_psi, h_vw, h_aX, h_bX, h_psiX, h_vwX;
+  int *d_sta = nullptr;
+  // profiling
+  bool prof = false;
+  struct Ev { int id; hipEvent_t a, b; };
+  std::vector<Ev> evs;
+  std::vector<hipEvent_t> pool;
+  double kms[K_COUNT] = {0};
+  long kcount[K_COUNT] = {0};
+  // step sequence
+  std::vector<KernelId> seq;
+};
+
+namespace {
+
+int fail(fd3d_handle *h, int code, const std::string &msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail((h), e_ == hipErrorOutOfMemory ? FD3D_E_NOMEM : FD3D_E_HIP,        \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                \
+  } while (0)
+
+Dims make_dims(const fd3d_config &c) {
+  Dims g{};
+  g.nxt = c.nxt; g.nyt = c.nyt; g.nzt = c.nzt; g.nabc = c.nabc;
+  g.fspace = c.fspace ? 1 : 0;
+  g.nfs = g.fspace ? c.nabc : 2;  // fd3d_init.f90:161-165
+  // row offset so that i = nabc+1 starts a 64-float (256 B) boundary
+  g.off0 = (64 - (c.nabc % 64)) % 64;
+  g.px = ((g.off0 + c.nxt + 63) / 64) * 64;
+  g.sy = g.px;
+  g.sz = g.px * c.nyt;
+  g.n3 = (long long)g.sz * c.nzt;
+  g.nxT = c.nxt - 2 * c.nabc;
+  g.nzT = c.nzt - g.nfs - c.nabc;
+  return g;
+}
+
+Coef make_coef(const fd3d_config &c) {
+  Coef k{};
+  k.dh = c.dh; k.dt = c.dt;
+  k.dth = c.dt / c.dh;
+  k.dthh = (0.5f * c.dt) / c.dh;
+  k.dht = c.dh / c.dt;
+  k.damp_s = c.damp_s;
+  k.c1 = 9.0f / 8.0f;
+  k.c2 = -1.0f / 24.0f;
+  return k;
+}
+
+// PML region table (allocation extents fd3d_deriv.f90:1324-1437).
+int build_regions(const fd3d_config &c, const Dims &g, Pml &pm, long long &nsplit, long long &nprof) {
+  const int nabc = c.nabc, nxt = c.nxt, nyt = c.nyt, nzt = c.nzt;
+  const int kb = nabc + 1, ke = nzt - g.nfs;
+  struct Spec { int ib, ie, jb, je_uw, je_v, je_xz, kb, ke, ztop, nzx, nzy; };
+  std::vector<Spec> sp = {
+      {2, nabc, nabc + 1, nyt, nyt - 1, nyt, kb, ke, 1, nyt - nabc, nyt - nabc - 1},
+      {nxt - nabc + 1, nxt - 1, nabc + 1, nyt, nyt - 1, nyt, kb, ke, g.fspace ? 0 : 1, nyt - nabc, nyt - nabc - 1},
+      {2, nxt - 1, 2, nabc, nabc, nabc, kb, ke, g.fspace ? 0 : 1, nabc - 1, nabc - 1},
+      {2, nxt - 1, 2, nyt, nyt - 1, nyt, 2, nabc, 0, 0, 0},
+  };
+  if (g.fspace) sp.push_back({2, nxt - 1, 2, nyt, nyt - 1, nyt, nzt - nabc + 1, nzt - 1, 0, 0, 0});
+  pm.nreg = (int)sp.size();
+  nsplit = 0;
+  nprof = 0;
+  int blk = 0;
+  for (int r = 0; r < pm.nreg; ++r) {
+    PmlRegion &R = pm.r[r];
+    const Spec &s = sp[r];
+    R.ib = s.ib; R.ie = s.ie; R.jb = s.jb; R.je_uw = s.je_uw; R.je_v = s.je_v; R.je_xz = s.je_xz;
+    R.kb = s.kb; R.ke = s.ke;
+    R.ni = s.ie - s.ib + 1; R.nj = s.je_uw - s.jb + 1; R.nk = s.ke - s.kb + 1;
+    if (R.ni <= 0 || R.nj <= 0 || R.nk <= 0) return -1;
+    R.ncell = R.ni * R.nj * R.nk;
+    R.ztop = s.ztop; R.nzx = s.nzx; R.nzy = s.nzy;
+    R.blk0 = blk;
+    blk += (R.ncell + kPmlBlock - 1) / kPmlBlock;
+    R.split_off = nsplit;
+    nsplit += (long long)kSplits * R.ncell;
+    R.prof_off = (int)nprof;
+    nprof += 4LL * (R.ni + 1) + 4LL * (R.nj + 1) + 4LL * (R.nk + 1);
+  }
+  pm.nblk = blk;
+  return 0;
+}
+
+// init_pml damping profiles (fd3d_deriv.f90:1439-1491), stored as (1-w) and
+// (1+w) per profile, region by region, in the order the kernels read them.
+std::vector<float> build_profiles(const fd3d_config &c, const Dims &g, const Pml &pm) {
+  const int nabc = c.nabc, nxt = c.nxt;
+  const float omegaM = (c.pml_fact * c.pml_vp) / ((2.0f * c.dh) * (float)(nabc - 1));
+  std::vector<float> om(nabc + 1, 0), omR(nabc + 1, 0), omM(nabc + 1, 0), omRM(nabc + 1, 0);
+  const float scale = (0.5f * c.dt) * omegaM;
+  for (int i = 1; i <= nabc - 1; ++i) {
+    float q = (float)i / (float)(nabc - 1);
+    om[i] = scale * (((q * q) * q) * q);
+    omR[nabc - i] = om[i];
+    float qm = ((float)i - 0.5f) / (float)(nabc - 1);
+    omM[i] = scale * (((qm * qm) * qm) * qm);
+    omRM[nabc - i] = omM[i];
+  }
+  long long total = 0;
+  for (int r = 0; r < pm.nreg; ++r)
+    total += 4LL * (pm.r[r].ni + 1) + 4LL * (pm.r[r].nj + 1) + 4LL * (pm.r[r].nk + 1);
+  std::vector<float> out((size_t)total, 0.0f);
+  for (int r = 0; r < pm.nreg; ++r) {
+    const PmlRegion &R = pm.r[r];
+    std::vector<float> ox(R.ni + 1, 0), oxS(R.ni + 1, 0), oy(R.nj + 1, 0), oyS(R.nj + 1, 0),
+        oz(R.nk + 1, 0), ozS(R.nk + 1, 0);
+    for (int i = 1; i <= nabc - 1; ++i) {
+      switch (r) {
+        case 0: ox[i] = omR[i]; oxS[i] = omRM[i]; break;
+        case 1: ox[i] = omM[i]; oxS[i] = om[i]; break;
+        case 2:
+          oy[i] = omR[i]; ox[i] = omR[i]; ox[nxt - i - 1] = omM[i];
+          oyS[i] = omRM[i]; oxS[i] = omRM[i]; oxS[nxt - i - 1] = om[i];
+          break;
+        case 3:
+          oz[i] = omR[i]; ox[i] = omR[i]; ox[nxt - i - 1] = omM[i]; oy[i] = omR[i];
+          ozS[i] = omRM[i]; oxS[i] = omRM[i]; oxS[nxt - i - 1] = om[i]; oyS[i] = omRM[i];
+          break;
+        default:
+          oz[i] = omM[i]; ox[i] = omR[i]; ox[nxt - i - 1] = omM[i]; oy[i] = omR[i];
+          ozS[i] = om[i]; oxS[i] = omRM[i]; oxS[nxt - i - 1] = om[i]; oyS[i] = omRM[i];
+          break;
+      }
+    }
+    // regions 1-2 read omegay/omegaz where later regions have yS/zS profiles,
+    // region 3 reads omegaz (pml_xyz, fd3d_deriv.f90:2398-2455, 2785-2815)
+    if (r <= 1) { oyS = oy; ozS = oz; }
+    if (r == 2) ozS = oz;
+    float *dst = out.data() + R.prof_off;
+    auto put = [&](const std::vector<float> &o) {
+      for (size_t n = 0; n < o.size(); ++n) *dst++ = 1.0f - o[n];
+      for (size_t n = 0; n < o.size(); ++n) *dst++ = 1.0f + o[n];
+    };
+    put(ox); put(oxS); put(oy); put(oyS); put(oz); put(ozS);
+  }
+  return out;
+}
+
+template <typename T>
+int dalloc(fd3d_handle *h, T **ptr, size_t n) {
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    hipGetLastError();
+    return fail(h, e == hipErrorOutOfMemory ? FD3D_E_NOMEM : FD3D_E_HIP,
+                std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  h->allocs.push_back(p);
+  h->bytes += n * sizeof(T);
+  *ptr = static_cast<T *>(p);
+  return 0;
+}
+
+int validate(const fd3d_config *c) {
+  if (!c) return FD3D_E_INVALID;
+  if (c->nabc < 3) return FD3D_E_INVALID;
+  const int nfs = c->fspace ? c->nabc : 2;
+  if (c->nxt < 2 * c->nabc + 4 || c->nyt < c->nabc + 4 || c->nzt < c->nabc + nfs + 4) return FD3D_E_INVALID;
+  if (!(c->dh > 0) || !(c->dt > 0)) return FD3D_E_INVALID;
+  if (c->friction != FD3D_SW && c->friction != FD3D_FVW) return FD3D_E_INVALID;
+  if (c->nstations < 0 || (c->nstations > 0 && !c->stations)) return FD3D_E_INVALID;
+  for (int s = 0; s < c->nstations; ++s) {
+    const int *t = c->stations + 3 * s;
+    if (t[0] < 1 || t[0] > c->nxt || t[1] < 1 || t[1] > c->nyt || t[2] < 1 || t[2] > c->nzt)
+      return FD3D_E_INVALID;
+  }
+  Dims g = make_dims(*c);
+  if (g.n3 >= (1LL << 31)) return FD3D_E_INVALID;  // 32-bit cell indices
+  return 0;
+}
+
+size_t footprint(const fd3d_config &c) {
+  Dims g = make_dims(c);
+  Pml pm{};
+  long long nsplit = 0, nprof = 0;
+  if (build_regions(c, g, pm, nsplit, nprof) != 0) return 0;
+  const size_t n2 = (size_t)c.nxt * c.nzt;
+  const int ring = c.ring_steps > 0 ? c.ring_steps : 64;
+  size_t b = 12 * (size_t)g.n3 * 4;                 // 3D fields
+  b += (size_t)nsplit * 4 + (size_t)nprof * 4;      // PML
+  b += n2 * 4 * 33 + n2 * 8 * 8;                    // fault plane
+  b += (size_t)ring * FD3D_NOUT * g.nxT * g.nzT * 4 + (size_t)ring * c.nstations * 3 * 4;
+  return b;
+}
+
+void record(fd3d_handle *h, KernelId id, bool start, hipEvent_t &ev) {
+  (void)id;
+  (void)start;
+  if (h->pool.empty()) {
+    hipEventCreate(&ev);
+  } else {
+    ev = h->pool.back();
+    h->pool.pop_back();
+  }
+  hipEventRecord(ev, h->stream);
+}
+
+int launch_one(fd3d_handle *h, KernelId id) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (h->prof) record(h, id, true, a);
+  hipError_t e = launch(id, h->p, h->stream);
+  if (e != hipSuccess) return fail(h, FD3D_E_HIP, std::string("launch ") + kKernelNames[id] + ": " + hipGetErrorString(e));
+  if (h->prof) {
+    record(h, id, false, b);
+    h->evs.push_back({(int)id, a, b});
+  }
+  return 0;
+}
+
+void harvest(fd3d_handle *h) {
+  for (auto &e : h->evs) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+      h->kms[e.id] += ms;
+      h->kcount[e.id] += 1;
+    }
+    h->pool.push_back(e.a);
+    h->pool.push_back(e.b);
+  }
+  h->evs.clear();
+}
+
+int check_ctl(fd3d_handle *h) {
+  Ctl c{};
+  HIPCHK(h, hipMemcpyAsync(&c, h->p.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (c.newton_capped > 0)
+    return fail(h, FD3D_E_NEWTON, "rate-and-state Newton solve hit the iteration cap at " +
+                                      std::to_string(c.newton_capped) + " node-steps");
+  if (c.nonfinite) return fail(h, FD3D_E_NONFINITE, "non-finite slip rate in the fault outputs");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t fd3d_footprint(const fd3d_config *cfg) {
+  if (validate(cfg) != 0) return 0;
+  return footprint(*cfg);
+}
+
+int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
+  if (!out) return FD3D_E_INVALID;
+  *out = nullptr;
+  int v = validate(cfg);
+  if (v != 0) return v;
+  fd3d_handle *h = new fd3d_handle();
+  h->cfg = *cfg;
+  if (cfg->nstations > 0) h->sta.assign(cfg->stations, cfg->stations + 3 * cfg->nstations);
+  h->cfg.stations = h->sta.data();
+  if (h->cfg.ring_steps <= 0) h->cfg.ring_steps = 64;
+  if (h->cfg.newton_max <= 0) h->cfg.newton_max = 200;
+  Problem &p = h->p;
+  p.g = make_dims(*cfg);
+  p.c = make_coef(*cfg);
+  long long nsplit = 0, nprof = 0;
+  if (build_regions(*cfg, p.g, p.pml, nsplit, nprof) != 0) {
+    delete h;
+    return FD3D_E_INVALID;
+  }
+  int rc = 0;
+  auto bail = [&](int code) {
+    fd3d_destroy(h);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(FD3D_E_HIP);
+  const size_t n3 = (size_t)p.g.n3, n2 = (size_t)cfg->nxt * cfg->nzt;
+  float **f3[] = {&p.f.u, &p.f.v, &p.f.w, &p.f.xx, &p.f.yy, &p.f.zz, &p.f.xy, &p.f.yz, &p.f.xz,
+                  &p.f.lam, &p.f.mu, &p.f.d};
+  for (auto pp : f3)
+    if ((rc = dalloc(h, pp, n3))) return bail(rc);
+  Fault &fl = p.fl;
+  float **f2[] = {&fl.tx, &fl.tz, &fl.v1t, &fl.avdx, &fl.avdz, &fl.RFx, &fl.RFz, &fl.au1, &fl.av1,
+                  &fl.aw1, &fl.uZ, &fl.wX, &fl.tabsX, &fl.tabsZ, &fl.T0X, &fl.T0Z, &fl.Dc, &fl.coh,
+                  &fl.peakX, &fl.dynX, &fl.DcX, &fl.peakZ, &fl.dynZ, &fl.DcZ, &fl.ruptime, &fl.rise,
+                  &fl.sliptime, &fl.slipX, &fl.slipZ, &fl.schangeX, &fl.schangeZ, &fl.distX, &fl.distZ};
+  for (auto pp : f2)
+    if ((rc = dalloc(h, pp, n2))) return bail(rc);
+  double **d2[] = {&fl.aX, &fl.bX, &fl.psiX, &fl.vwX, &fl.aZ, &fl.bZ, &fl.psiZ, &fl.vwZ};
+  for (auto pp : d2)
+    if ((rc = dalloc(h, pp, n2))) return bail(rc);
+  if ((rc = dalloc(h, &p.pml.splits, (size_t)nsplit))) return bail(rc);
+  if ((rc = dalloc(h, &p.pml.prof, (size_t)nprof))) return bail(rc);
+  const size_t slab = (size_t)p.g.nxT * p.g.nzT;
+  if ((rc = dalloc(h, &p.out.ring, (size_t)h->cfg.ring_steps * FD3D_NOUT * slab))) return bail(rc);
+  if ((rc = dalloc(h, &p.out.seis, (size_t)h->cfg.ring_steps * std::max(1, cfg->nstations) * 3))) return bail(rc);
+  if ((rc = dalloc(h, &h->d_sta, (size_t)std::max(1, cfg->nstations) * 3))) return bail(rc);
+  if ((rc = dalloc(h, &p.ctl, 1))) return bail(rc);
+  if (cfg->nstations > 0 &&
+      hipMemcpy(h->d_sta, h->sta.data(), h->sta.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(FD3D_E_HIP);
+  p.out.sta = h->d_sta;
+  p.out.nsta = cfg->nstations;
+  p.out.ring_steps = h->cfg.ring_steps;
+  // zero everything once (pads included); the prologue zeroes state again
+  for (auto pp : f3)
+    if (hipMemsetAsync(*pp, 0, n3 * sizeof(float), h->stream) != hipSuccess) return bail(FD3D_E_HIP);
+  std::vector<float> prof = build_profiles(*cfg, p.g, p.pml);
+  if (hipMemcpy(p.pml.prof, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(FD3D_E_HIP);
+  FvwScalars &fv = p.fv;
+  fv.Sn = cfg->Sn; fv.uini = cfg->uini; fv.wini = cfg->wini; fv.f0 = cfg->f0; fv.v0 = cfg->v0;
+  fv.fw = cfg->fw; fv.hx0 = cfg->hx0; fv.hz0 = cfg->hz0; fv.perturb = cfg->perturb; fv.RR2 = cfg->RR2;
+  fv.TT2 = cfg->TT2; fv.strinixI = cfg->strinixI; fv.newton_max = h->cfg.newton_max;
+  fv.fvw = cfg->friction == FD3D_FVW; fv.nucleation = fv.fvw && cfg->nucleation;
+  // kernel sequence of one step (fd3d_theo.f90:208-542)
+  h->seq = {K_VEL_INTERIOR, K_VEL_FAULT, K_PML_VEL};
+  if (!p.g.fspace) { h->seq.push_back(K_FUVW_UV); h->seq.push_back(K_FUVW_W); }
+  h->seq.push_back(K_STR_INTERIOR); h->seq.push_back(K_STR_FAULT); h->seq.push_back(K_PML_STR);
+  if (!p.g.fspace) h->seq.push_back(K_FRES);
+  h->seq.push_back(K_TRACTION);
+  if (!p.g.fspace) h->seq.push_back(K_TRACTION_MIRROR);
+  h->seq.push_back(K_TABS); h->seq.push_back(K_FRICTION); h->seq.push_back(K_OUTPUTS);
+  if (fv.nucleation) h->seq.push_back(K_NUCLEATION);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(FD3D_E_HIP);
+  *out = h;
+  return 0;
+}
+
+static int upload3(fd3d_handle *h, float *dst, const float *src) {
+  const Dims &g = h->p.g;
+  HIPCHK(h, hipMemcpy2D(dst + g.off0, (size_t)g.px * sizeof(float), src, (size_t)g.nxt * sizeof(float),
+                        (size_t)g.nxt * sizeof(float), (size_t)g.nyt * g.nzt, hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int download3(fd3d_handle *h, float *dst, const float *src) {
+  const Dims &g = h->p.g;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpy2D(dst, (size_t)g.nxt * sizeof(float), src + g.off0, (size_t)g.px * sizeof(float),
+                        (size_t)g.nxt * sizeof(float), (size_t)g.nyt * g.nzt, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int fd3d_upload_medium(fd3d_handle *h, const float *lam1, const float *mu1, const float *d1) {
+  if (!h || !lam1 || !mu1 || !d1) return fail(h, FD3D_E_INVALID, "null argument");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  // CFL check on the largest P velocity (fd3d_init.f90:274-278)
+  const size_t n = (size_t)h->cfg.nxt * h->cfg.nyt * h->cfg.nzt;
+  float vmax = 0.0f;
+  for (size_t q = 0; q < n; ++q) {
+    if (!(d1[q] > 0.0f)) return fail(h, FD3D_E_INVALID, "density must be positive");
+    const float vp = std::sqrt((lam1[q] + 2.0f * mu1[q]) / d1[q]);
+    if (vp > vmax) vmax = vp;
+  }
+  const float cfl = vmax * h->cfg.dt / h->cfg.dh;
+  if (cfl > 0.25f) return fail(h, FD3D_E_CFL, "numerically unstable: CFL = " + std::to_string(cfl));
+  int rc;
+  if ((rc = upload3(h, h->p.f.lam, lam1))) return rc;
+  if ((rc = upload3(h, h->p.f.mu, mu1))) return rc;
+  if ((rc = upload3(h, h->p.f.d, d1))) return rc;
+  h->medium_ok = true;
+  return 0;
+}
+
+int fd3d_upload_fault_sw(fd3d_handle *h, const float *striniX, const float *striniZ, const float *peak_xz,
+                         const float *dyn_xz, const float *Dc, const float *coh) {
+  if (!h || !striniX || !striniZ || !peak_xz || !dyn_xz || !Dc || !coh) return fail(h, FD3D_E_INVALID, "null argument");
+  if (h->cfg.friction != FD3D_SW) return fail(h, FD3D_E_INVALID, "handle is rate-and-state");
+  const size_t n2 = (size_t)h->cfg.nxt * h->cfg.nzt;
+  h->h_striniX.assign(striniX, striniX + n2);
+  h->h_striniZ.assign(striniZ, striniZ + n2);
+  h->h_peak.assign(peak_xz, peak_xz + n2);
+  h->h_dyn.assign(dyn_xz, dyn_xz + n2);
+  h->h_Dc.assign(Dc, Dc + n2);
+  h->h_coh.assign(coh, coh + n2);
+  h->fault_ok = true;
+  return 0;
+}
+
+int fd3d_upload_fault_fvw(fd3d_handle *h, const double *Dc, const double *striniX, const double *striniZ,
+                          const double *a, const double *b, const double *psi, const double *vw,
+                          const double *aX, const double *bX, const double *psiX, const double *vwX) {
+  if (!h || !Dc || !striniX || !striniZ || !a || !b || !psi || !vw || !aX || !bX || !psiX || !vwX)
+    return fail(h, FD3D_E_INVALID, "null argument");
+  if (h->cfg.friction != FD3D_FVW) return fail(h, FD3D_E_INVALID, "handle is slip weakening");
+  const size_t n2 = (size_t)h->cfg.nxt * h->cfg.nzt;
+  auto tof = [&](const double *s) { std::vector<float> o(n2); for (size_t q = 0; q < n2; ++q) o[q] = (float)s[q]; return o; };
+  h->h_Dc = tof(Dc);
+  h->h_striniX = tof(striniX);
+  h->h_striniZ = tof(striniZ);
+  h->h_coh.assign(n2, 0.0f);
+  h->h_a.assign(a, a + n2); h->h_b.assign(b, b + n2); h->h_psi.assign(psi, psi + n2); h->h_vw.assign(vw, vw + n2);
+  h->h_aX.assign(aX, aX + n2); h->h_bX.assign(bX, bX + n2); h->h_psiX.assign(psiX, psiX + n2);
+  h->h_vwX.assign(vwX, vwX + n2);
+  h->fault_ok = true;
+  return 0;
+}
+
+int fd3d_init_state(fd3d_handle *h) {
+  if (!h) return FD3D_E_INVALID;
+  if (!h->medium_ok || !h->fault_ok) return fail(h, FD3D_E_STATE, "upload the medium and the fault first");
+  Problem &p = h->p;
+  const Dims &g = p.g;
+  const size_t n3 = (size_t)g.n3;
+  const int nxt = h->cfg.nxt, nzt = h->cfg.nzt;
+  const size_t n2 = (size_t)nxt * nzt;
+  hipStream_t s = h->stream;
+  float *f3[] = {p.f.u, p.f.v, p.f.w, p.f.xx, p.f.yy, p.f.zz, p.f.xy, p.f.yz, p.f.xz};
+  for (auto a : f3) HIPCHK(h, hipMemsetAsync(a, 0, n3 * sizeof(float), s));
+  Fault &fl = p.fl;
+  float *z2[] = {fl.tx, fl.tz, fl.v1t, fl.avdx, fl.avdz, fl.RFx, fl.RFz, fl.au1, fl.av1, fl.aw1, fl.uZ,
+                 fl.wX, fl.rise, fl.slipX, fl.slipZ, fl.schangeX, fl.schangeZ, fl.distX, fl.distZ};
+  for (auto a : z2) HIPCHK(h, hipMemsetAsync(a, 0, n2 * sizeof(float), s));
+  HIPCHK(h, fill_f32(fl.ruptime, 1.e4f, (long long)n2, s));
+  HIPCHK(h, fill_f32(fl.sliptime, 1.e4f, (long long)n2, s));
+  const float tabs0 = p.fv.fvw ? 1.0f : 0.0f;  // fd3d_theo.f90:72,79-80
+  HIPCHK(h, fill_f32(fl.tabsX, tabs0, (long long)n2, s));
+  HIPCHK(h, fill_f32(fl.tabsZ, tabs0, (long long)n2, s));
+  long long nsplit = 0;
+  for (int r = 0; r < p.pml.nreg; ++r) nsplit += (long long)kSplits * p.pml.r[r].ncell;
+  HIPCHK(h, hipMemsetAsync(p.pml.splits, 0, (size_t)nsplit * sizeof(float), s));
+  HIPCHK(h, hipMemsetAsync(p.ctl, 0, sizeof(Ctl), s));
+  // interp_fric (fd3d_deriv.f90:3150-3190) on the host, float arithmetic
+  std::vector<float> peakX(n2, 0), dynX(n2, 0), DcX(n2, 0), T0X(n2, 0);
+  auto I = [&](int i, int k) { return (size_t)(i - 1) + (size_t)nxt * (size_t)(k - 1); };
+  const bool fvw = p.fv.fvw;
+  for (int k = 2; k <= nzt - 1; ++k)
+    for (int i = 2; i <= nxt - 1; ++i) {
+      size_t c = I(i, k), im = I(i - 1, k), km = I(i, k - 1), ikm = I(i - 1, k - 1);
+      if (!fvw) {
+        peakX[c] = (((h->h_peak[c] + h->h_peak[im]) + h->h_peak[km]) + h->h_peak[ikm]) / 4.0f;
+        dynX[c] = (((h->h_dyn[c] + h->h_dyn[im]) + h->h_dyn[km]) + h->h_dyn[ikm]) / 4.0f;
+        DcX[c] = (((h->h_Dc[c] + h->h_Dc[im]) + h->h_Dc[km]) + h->h_Dc[ikm]) / 4.0f;
+      }
+      T0X[c] = (((h->h_striniX[c] + h->h_striniX[im]) + h->h_striniX[km]) + h->h_striniX[ikm]) / 4.0f;
+    }
+  auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+    HIPCHK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return 0;
+  };
+  int rc;
+  const size_t fb = n2 * sizeof(float), db = n2 * sizeof(double);
+  std::vector<float> zeros(n2, 0.0f);
+  if ((rc = up(fl.T0X, T0X.data(), fb))) return rc;
+  if ((rc = up(fl.T0Z, h->h_striniZ.data(), fb))) return rc;
+  if ((rc = up(fl.Dc, h->h_Dc.data(), fb))) return rc;
+  if ((rc = up(fl.coh, h->h_coh.data(), fb))) return rc;
+  if (!fvw) {
+    if ((rc = up(fl.peakX, peakX.data(), fb))) return rc;
+    if ((rc = up(fl.dynX, dynX.data(), fb))) return rc;
+    if ((rc = up(fl.DcX, DcX.data(), fb))) return rc;
+    if ((rc = up(fl.peakZ, h->h_peak.data(), fb))) return rc;
+    if ((rc = up(fl.dynZ, h->h_dyn.data(), fb))) return rc;
+    if ((rc = up(fl.DcZ, h->h_Dc.data(), fb))) return rc;
+  } else {
+    if ((rc = up(fl.aZ, h->h_a.data(), db))) return rc;
+    if ((rc = up(fl.bZ, h->h_b.data(), db))) return rc;
+    if ((rc = up(fl.psiZ, h->h_psi.data(), db))) return rc;
+    if ((rc = up(fl.vwZ, h->h_vw.data(), db))) return rc;
+    if ((rc = up(fl.aX, h->h_aX.data(), db))) return rc;
+    if ((rc = up(fl.bX, h->h_bX.data(), db))) return rc;
+    if ((rc = up(fl.psiX, h->h_psiX.data(), db))) return rc;
+    if ((rc = up(fl.vwX, h->h_vwX.data(), db))) return rc;
+  }
+  HIPCHK(h, hipStreamSynchronize(s));
+  h->steps = 0;
+  h->inited = true;
+  return 0;
+}
+
+int fd3d_step(fd3d_handle *h, int nsteps, float *out, float *seis) {
+  if (!h) return FD3D_E_INVALID;
+  if (!h->inited) return fail(h, FD3D_E_STATE, "fd3d_init_state first");
+  if (nsteps < 0) return fail(h, FD3D_E_INVALID, "negative step count");
+  const int ring = h->cfg.ring_steps;
+  const size_t slab = (size_t)h->p.g.nxT * h->p.g.nzT;
+  const int nsta = h->cfg.nstations;
+  int done = 0;
+  while (done < nsteps) {
+    const int pos = h->steps % ring;
+    const int chunk = std::min(nsteps - done, ring - pos);
+    for (int s = 0; s < chunk; ++s) {
+      for (KernelId id : h->seq) {
+        int rc = launch_one(h, id);
+        if (rc) return rc;
+      }
+    }
+    if (out) {
+      HIPCHK(h, hipMemcpyAsync(out + (size_t)done * FD3D_NOUT * slab, h->p.out.ring + (size_t)pos * FD3D_NOUT * slab,
+                               (size_t)chunk * FD3D_NOUT * slab * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    }
+    if (seis && nsta > 0) {
+      HIPCHK(h, hipMemcpyAsync(seis + (size_t)done * nsta * 3, h->p.out.seis + (size_t)pos * nsta * 3,
+                               (size_t)chunk * nsta * 3 * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    }
+    if (out || seis) HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->steps += chunk;
+    done += chunk;
+  }
+  if (out || seis) return check_ctl(h);
+  return 0;
+}
+
+int fd3d_sync(fd3d_handle *h) {
+  if (!h) return FD3D_E_INVALID;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return check_ctl(h);
+}
+
+int fd3d_steps_done(const fd3d_handle *h) { return h ? h->steps : 0; }
+
+int fd3d_download_final(fd3d_handle *h, float *out7) {
+  if (!h || !out7) return fail(h, FD3D_E_INVALID, "null argument");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const size_t n2 = (size_t)h->cfg.nxt * h->cfg.nzt;
+  std::vector<float> t0x(n2), t0z(n2), rise(n2), schx(n2), schz(n2);
+  Fault &fl = h->p.fl;
+  HIPCHK(h, hipMemcpy(out7, fl.ruptime, n2 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(rise.data(), fl.rise, n2 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(out7 + 2 * n2, fl.sliptime, n2 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(out7 + 3 * n2, fl.slipX, n2 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(out7 + 4 * n2, fl.slipZ, n2 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(schx.data(), fl.schangeX, n2 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(schz.data(), fl.schangeZ, n2 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(t0x.data(), fl.T0X, n2 * 4, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(t0z.data(), fl.T0Z, n2 * 4, hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < n2; ++q) {
+    out7[n2 + q] = rise[q] - out7[q];          // fd3d_theo.f90:658
+    out7[5 * n2 + q] = schx[q] - t0x[q];       // :607
+    out7[6 * n2 + q] = schz[q] - t0z[q];       // :606
+  }
+  return 0;
+}
+
+static float *field_ptr(fd3d_handle *h, int which) {
+  Fields &f = h->p.f;
+  switch (which) {
+    case FD3D_U: return f.u; case FD3D_V: return f.v; case FD3D_W: return f.w;
+    case FD3D_XX: return f.xx; case FD3D_YY: return f.yy; case FD3D_ZZ: return f.zz;
+    case FD3D_XY: return f.xy; case FD3D_YZ: return f.yz; case FD3D_XZ: return f.xz;
+    default: return nullptr;
+  }
+}
+
+int fd3d_download_field(fd3d_handle *h, int which, float *out) {
+  if (!h || !out) return fail(h, FD3D_E_INVALID, "null argument");
+  float *src = field_ptr(h, which);
+  if (!src) return fail(h, FD3D_E_INVALID, "unknown field");
+  return download3(h, out, src);
+}
+
+int fd3d_upload_field(fd3d_handle *h, int which, const float *in) {
+  if (!h || !in) return fail(h, FD3D_E_INVALID, "null argument");
+  float *dst = field_ptr(h, which);
+  if (!dst) return fail(h, FD3D_E_INVALID, "unknown field");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return upload3(h, dst, in);
+}
+
+int fd3d_set_profiling(fd3d_handle *h, int on) {
+  if (!h) return FD3D_E_INVALID;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  harvest(h);
+  for (int k = 0; k < K_COUNT; ++k) { h->kms[k] = 0; h->kcount[k] = 0; }
+  h->prof = on != 0;
+  return 0;
+}
+
+int fd3d_kernel_times(fd3d_handle *h, int n, const char **names, double *ms, long *launches) {
+  if (!h) return FD3D_E_INVALID;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  harvest(h);
+  int c = 0;
+  for (int k = 0; k < K_COUNT && c < n; ++k) {
+    if (h->kcount[k] == 0) continue;
+    if (names) names[c] = kKernelNames[k];
+    if (ms) ms[c] = h->kms[k];
+    if (launches) launches[c] = h->kcount[k];
+    ++c;
+  }
+  return c;
+}
+
+double fd3d_step_bytes(const fd3d_handle *h, long *bulk_cells, long *pml_cells) {
+  if (!h) return 0.0;
+  const Dims &g = h->p.g;
+  long pml = 0;
+  for (int r = 0; r < h->p.pml.nreg; ++r) pml += h->p.pml.r[r].ncell;
+  // bulk: the 4th-order interior and the near-fault rows j = nyt-1, nyt
+  const long bulk = (long)(g.nxt - 2 * g.nabc) * (long)(g.nyt - g.nabc) * (long)(g.nzt - g.nfs - g.nabc);
+  if (bulk_cells) *bulk_cells = bulk;
+  if (pml_cells) *pml_cells = pml;
+  return 120.0 * (double)bulk + 276.0 * (double)pml;
+}
+
+const char *fd3d_last_error(const fd3d_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+void fd3d_destroy(fd3d_handle *h) {
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto &e : h->evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
+  for (auto e : h->pool) hipEventDestroy(e);
+  for (void *p : h->allocs) hipFree(p);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+}  // extern "C"

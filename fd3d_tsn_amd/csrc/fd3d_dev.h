@@ -1,0 +1,121 @@
+// Device-side data layout of one rupture problem and the launch wrappers
+// implemented in kernels.hip.
+//
+// HBM layout.  Every 3D field (u, v, w, xx, yy, zz, xy, yz, xz, lam1, mu1,
+// d1) is its own float array in the reference's column-major order with a
+// padded x pitch: element (i,j,k), 1-based, lives at
+//   off0 + (i-1) + px*((j-1) + nyt*(k-1)),
+// where off0 shifts each row so that the first interior point i = nabc+1
+// starts a 256-byte boundary and px is a multiple of 64 floats.  A
+// wavefront (64 lanes along x) then reads whole 128-byte lines.  The fault
+// plane arrays (nxt, nzt) are unpadded.  PML split fields live in one pool,
+// region by region, split by split, each split a dense (ni, nj, nk) box.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fd3d {
+
+struct Dims {
+  int nxt, nyt, nzt, nabc, nfs, fspace;
+  int px, off0;
+  int sy, sz;        // strides of j and k in a 3D array
+  long long n3;      // floats per padded 3D array
+  int nxT, nzT;      // output slab (nxt-2nabc, nzt-nfs-nabc)
+};
+
+__host__ __device__ inline int I3(const Dims &g, int i, int j, int k) {
+  return g.off0 + (i - 1) + g.sy * (j - 1) + g.sz * (k - 1);
+}
+__host__ __device__ inline int I2(const Dims &g, int i, int k) { return (i - 1) + g.nxt * (k - 1); }
+
+// Float constants formed on the host exactly as the reference forms them.
+struct Coef {
+  float dh, dt;
+  float dth;   // dt/dh
+  float dthh;  // (0.5*dt)/dh, the near-fault shear kernels (fd3d_deriv.f90:696,902)
+  float dht;   // dh/dt
+  float damp_s, c1, c2;
+};
+
+struct Fields {
+  float *u, *v, *w, *xx, *yy, *zz, *xy, *yz, *xz, *lam, *mu, *d;
+};
+
+struct Fault {
+  // traction_com
+  float *tx, *tz, *v1t, *avdx, *avdz, *RFx, *RFz, *au1, *av1, *aw1;
+  // friction_com
+  float *uZ, *wX, *tabsX, *tabsZ, *T0X, *T0Z, *Dc, *coh;
+  float *peakX, *dynX, *DcX, *peakZ, *dynZ, *DcZ;
+  double *aX, *bX, *psiX, *vwX, *aZ, *bZ, *psiZ, *vwZ;
+  // source_com and the loop-local slip paths
+  float *ruptime, *rise, *sliptime, *slipX, *slipZ, *schangeX, *schangeZ, *distX, *distZ;
+};
+
+struct FvwScalars {
+  double Sn, uini, wini, f0, v0, fw, hx0, hz0, perturb, RR2, TT2, strinixI;
+  int newton_max, fvw, nucleation;
+};
+
+struct PmlRegion {
+  int ib, ie, jb, je_uw, je_v, je_xz, kb, ke;
+  int ni, nj, nk, ncell;
+  int ztop, nzx, nzy;      // zero xz/yz splits at k2 = nk for j2 <= nzx / nzy
+  int blk0;                // first block of this region in the fused PML launch
+  long long split_off;     // offset of split 0 in the pool (floats)
+  int prof_off;            // offset of the profile block in the profile pool
+};
+
+constexpr int kMaxRegions = 5;
+constexpr int kSplits = 24;
+constexpr int kPmlBlock = 256;
+
+struct Pml {
+  int nreg, nblk;
+  PmlRegion r[kMaxRegions];
+  float *splits;
+  // per region 12 profile arrays: for p in (ox, oxS, oy, oyS, oz, ozS):
+  // (1 - omega) then (1 + omega), 1-based, lengths ni+1 / nj+1 / nk+1
+  float *prof;
+};
+
+struct Ctl {
+  int it;              // steps taken since init (time = it*dt)
+  int newton_capped;   // Newton solves that hit the cap
+  int nonfinite;       // NaN/Inf seen in the slip-rate outputs
+  int pad;
+};
+
+struct Outputs {
+  float *ring;         // [ring][FD3D_NOUT][nxT*nzT]
+  float *seis;         // [ring][nsta][3]
+  const int *sta;      // [nsta][3] 1-based
+  int ring_steps, nsta;
+};
+
+struct Problem {
+  Dims g;
+  Coef c;
+  Fields f;
+  Fault fl;
+  FvwScalars fv;
+  Pml pml;
+  Outputs out;
+  Ctl *ctl;
+};
+
+enum KernelId {
+  K_VEL_INTERIOR = 0, K_VEL_FAULT, K_PML_VEL, K_FUVW_UV, K_FUVW_W,
+  K_STR_INTERIOR, K_STR_FAULT, K_PML_STR, K_FRES,
+  K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_NUCLEATION,
+  K_COUNT
+};
+extern const char *kKernelNames[K_COUNT];
+
+// Launch one kernel of the step on a stream; returns hipSuccess or the launch error.
+hipError_t launch(KernelId id, const Problem &p, hipStream_t s);
+// Fill helpers used by the prologue.
+hipError_t fill_f32(float *p, float v, long long n, hipStream_t s);
+
+}  // namespace fd3d

@@ -1,0 +1,161 @@
+/*
+ * fd3d_tsn.h -- C ABI of the MI355X time-stepping core for FD3D_TSN.
+ *
+ * Replaces the reference's time loop SUBROUTINE fd3d() (src/fd3d_theo.f90:22,
+ * called from src/dynamicsolver.f90:66) together with the kernels it calls
+ * in src/fd3d_deriv.f90.  The reference passes all state through Fortran
+ * MODULE arrays (src/fd3d_init.f90:15-124); here the caller hands the same
+ * arrays over once as plain pointers, in the same column-major layout
+ * (x fastest, then y, then z; element (i,j,k), 1-based, at
+ * (i-1) + nxt*((j-1) + nyt*(k-1))), and the library keeps the whole
+ * wavefield resident in HBM for the lifetime of the handle.
+ *
+ * Every function returns 0 on success or a negative FD3D_E* code; the
+ * message of the last failure on a handle is fd3d_last_error(h).
+ * A handle is used from one host thread at a time.  Calls on different
+ * handles are independent (each handle owns its HIP stream).
+ *
+ * Fortran binding: every entry point takes plain pointers and integers by
+ * value, so an interface block with bind(C) and type(c_ptr) / value
+ * arguments maps 1:1 (see fd3d_tsn_amd/driver/fd3d_tsn_mod.f90).
+ */
+#ifndef FD3D_TSN_H
+#define FD3D_TSN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FD3D_OK 0
+#define FD3D_E_INVALID (-1)      /* bad configuration or argument          */
+#define FD3D_E_CFL (-2)          /* Vp*dt/dh > 0.25 (fd3d_init.f90:274-278) */
+#define FD3D_E_NOMEM (-3)        /* device memory exhausted                */
+#define FD3D_E_HIP (-4)          /* HIP runtime error                      */
+#define FD3D_E_STATE (-5)        /* call out of order (e.g. step before init) */
+#define FD3D_E_NEWTON (-6)       /* rate-and-state Newton hit its iteration cap */
+#define FD3D_E_NONFINITE (-7)    /* NaN/Inf detected in the fault outputs  */
+
+enum fd3d_friction { FD3D_SW = 0, FD3D_FVW = 1 };
+
+/* Per-step fault output slabs, in the order of the reference's .res streams
+ * (fd3d_theo.f90:548-553).  Each slab is nxtT*nztT floats, x fastest,
+ * covering i in [nabc+1, nxt-nabc], k in [nabc+1, nzt-nfs]. */
+enum fd3d_output {
+  FD3D_OUT_SLIPRATE_X = 0,   /* result/sliprateX.res    */
+  FD3D_OUT_SLIPRATE_Z = 1,   /* result/sliprateZ.res    */
+  FD3D_OUT_SHEAR_X = 2,      /* result/shearstressX.res */
+  FD3D_OUT_SHEAR_Z = 3,      /* result/shearstressZ.res */
+  FD3D_OUT_PSI = 4,          /* result/psi.res (rate-and-state only) */
+  FD3D_NOUT = 5
+};
+
+/* 3D wavefield ids for fd3d_download_field / fd3d_upload_field */
+enum fd3d_field {
+  FD3D_U = 0, FD3D_V, FD3D_W, FD3D_XX, FD3D_YY, FD3D_ZZ, FD3D_XY, FD3D_YZ, FD3D_XZ
+};
+
+/* Final 2D arrays (nxt*nzt each), fd3d_theo.f90:604-658 */
+enum fd3d_final {
+  FD3D_RUPTIME = 0, FD3D_RISE, FD3D_SLIPTIME, FD3D_SLIP_X, FD3D_SLIP_Z,
+  FD3D_STRESSDROP_X, FD3D_STRESSDROP_Z, FD3D_NFINAL
+};
+
+/* Problem definition: the values fd3d_init() reads from inputfd3d.dat
+ * (fd3d_init.f90:145-171) plus the macro choices of the reference build
+ * (-DFSPACE, -DFVW, -DTPV104). */
+typedef struct fd3d_config {
+  int nxt, nyt, nzt;      /* full grid, PML and ghost layers included          */
+  int nabc;               /* PML thickness (>= 3)                              */
+  int fspace;             /* 1: full space (nfs = nabc, z-high PML); 0: free surface (nfs = 2) */
+  int friction;           /* FD3D_SW or FD3D_FVW                               */
+  int nucleation;         /* FVW only: TPV104 smooth nucleation in time (fd3d_theo.f90:515-542) */
+  int nstations;          /* stations recorded every step (u, v, w)           */
+  const int *stations;    /* 3*nstations 1-based grid indices (staX, staY, staZ) */
+  float dh, dt;           /* grid step [m], time step [s]                      */
+  float damp_s;           /* Dalguer-Day damping near the fault                */
+  float pml_vp, pml_fact; /* PML: omegaM = pml_fact*pml_vp/(2*dh*(nabc-1))      */
+  /* rate-and-state scalars (friction_com, fd3d_init.f90:61), double precision */
+  double Sn, uini, wini, f0, v0, fw;
+  double hx0, hz0, perturb, RR2, TT2, strinixI;
+  int newton_max;         /* Newton iteration cap (0 -> 200); the reference has none */
+  int ring_steps;         /* device output ring length in steps (0 -> 64)      */
+} fd3d_config;
+
+typedef struct fd3d_handle fd3d_handle;
+
+/* Replaces the allocation half of fd3d() (fd3d_theo.f90:53-57) and
+ * init_pml's allocations (fd3d_deriv.f90:1319-1437).  Allocates every
+ * device array; fails with FD3D_E_NOMEM when HBM does not hold the
+ * problem.  Only checks the configuration; the CFL check needs the medium
+ * (fd3d_upload_medium). */
+int fd3d_create(const fd3d_config *cfg, fd3d_handle **out);
+
+/* Device bytes fd3d_create would allocate for cfg (admission control). */
+size_t fd3d_footprint(const fd3d_config *cfg);
+
+/* medium_com arrays lam1, mu1, d1 (fd3d_init.f90:262-264), nxt*nyt*nzt each.
+ * Returns FD3D_E_CFL when max(Vp)*dt/dh > 0.25 (fd3d_init.f90:274-278). */
+int fd3d_upload_medium(fd3d_handle *h, const float *lam1, const float *mu1, const float *d1);
+
+/* Slip-weakening fault inputs at Z-nodes, nxt*nzt each: striniX, striniZ,
+ * peak_xz, dyn_xz, Dc, coh (friction_com, fd3d_init.f90:191). */
+int fd3d_upload_fault_sw(fd3d_handle *h, const float *striniX, const float *striniZ,
+                         const float *peak_xz, const float *dyn_xz, const float *Dc,
+                         const float *coh);
+
+/* Rate-and-state inputs, nxt*nzt each (fd3d_init.f90:182-186): Dc, striniX,
+ * striniZ, then a, b, psi, vw at Z-nodes and aX, bX, psiX, vwX at X-nodes.
+ * Double precision: the Newton solve and state update run in FP64. */
+int fd3d_upload_fault_fvw(fd3d_handle *h, const double *Dc, const double *striniX,
+                          const double *striniZ, const double *a, const double *b,
+                          const double *psi, const double *vw, const double *aX,
+                          const double *bX, const double *psiX, const double *vwX);
+
+/* The prologue of fd3d() (fd3d_theo.f90:59-88): zero the wavefield and the
+ * fault state, build the PML profiles (init_pml) and the staggered fault
+ * parameters (interp_fric).  May be called again to restart. */
+int fd3d_init_state(fd3d_handle *h);
+
+/* Advance nsteps time steps (the body of fd3d_theo.f90:187-602).
+ * out:  NULL, or room for nsteps*FD3D_NOUT*nxtT*nztT floats, filled step by
+ *       step with the output slabs in fd3d_output order.
+ * seis: NULL, or room for nsteps*nstations*3 floats (u, v, w per station).
+ * Kernels run asynchronously on the handle's stream; the call returns after
+ * the last step's outputs have reached the host (or immediately when both
+ * pointers are NULL: use fd3d_sync). */
+int fd3d_step(fd3d_handle *h, int nsteps, float *out, float *seis);
+
+/* Wait for queued steps; reports FD3D_E_NEWTON / FD3D_E_NONFINITE. */
+int fd3d_sync(fd3d_handle *h);
+
+/* Steps taken since fd3d_init_state. */
+int fd3d_steps_done(const fd3d_handle *h);
+
+/* The arrays fd3d() leaves in source_com after the loop (fd3d_theo.f90:604-658),
+ * in fd3d_final order, each nxt*nzt: ruptime, rise (minus ruptime), sliptime,
+ * slipX, slipZ, schangeX-T0X, schangeZ-T0Z.  out7 holds 7*nxt*nzt floats. */
+int fd3d_download_final(fd3d_handle *h, float *out7);
+
+/* Copy one 3D field to / from the host (nxt*nyt*nzt floats, reference layout). */
+int fd3d_download_field(fd3d_handle *h, int which, float *out);
+int fd3d_upload_field(fd3d_handle *h, int which, const float *in);
+
+/* Per-kernel timing with HIP events on the handle's stream (0 = off).
+ * fd3d_kernel_times fills up to n entries of names/ms/launches for the
+ * kernels launched since profiling was switched on; returns the count. */
+int fd3d_set_profiling(fd3d_handle *h, int on);
+int fd3d_kernel_times(fd3d_handle *h, int n, const char **names, double *ms, long *launches);
+
+/* Algorithmic HBM bytes of one step of this problem: 120 B per bulk cell
+ * and 276 B per PML cell (fault plane excluded), and the cell counts. */
+double fd3d_step_bytes(const fd3d_handle *h, long *bulk_cells, long *pml_cells);
+
+const char *fd3d_last_error(const fd3d_handle *h);
+void fd3d_destroy(fd3d_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
