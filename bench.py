@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Benchmark of the FD3D_TSN time-stepping core on one MI355X.
+
+A step is one time step of the plain solver (velocity and stress sweeps,
+split-node fault boundary, per-step fault output slabs written to the
+device ring) on the single large slip-weakening rupture:
+1024x512x512 interior cells + 20-node PML (1064x532x552 = 312.5M cells),
+dh 50 m, dt 2 ms, full space, von Karman prestress.  Inputs are synthetic
+(seeded fields), resident in HBM before the timed region.
+
+Metric: Gcell-updates/s = nxt*nyt*nzt*steps / time (all cells, PML
+included, the SRL 2020 convention).  The JSON line also carries the
+roofline of the dominant kernel (algorithmic bytes per launch over its
+HIP-event-timed average duration) and the reference CPU build timed on a
+bounded sample on this host.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell of each sweep (FP32 words x 4)
+BYTES_PER_CELL = {
+    "vel_interior": 13 * 4,   # reads 6 stresses + rho + u, v, w; writes u, v, w
+    "str_interior": 17 * 4,   # reads u, v, w, lam, mu + 6 stresses; writes 6 stresses
+    "pml_vel": 28 * 4,        # 9 splits read+write, 6 stresses + rho read, 3 velocities written
+    "pml_str": 41 * 4,        # 15 splits read+write, 3 velocities + lam + mu read, 6 stresses written
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="large_sw", choices=["large_sw", "plain_sw", "fvw"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="time steps of the reference CPU build on the same grid")
+    return ap.parse_args()
+
+
+def workload(name):
+    from fd3d_tsn_amd import cases as C
+    if name == "large_sw":
+        return C.c2_large_sw()
+    if name == "plain_sw":
+        return C.c1_plain_sw()
+    return C.c3_fvw()
+
+
+def cells_of(case, kernel):
+    nxi = case.nxt - 2 * case.nabc
+    nyi = case.nyt - 2 - case.nabc
+    nzi = case.nzt - case.nfs - case.nabc
+    if kernel in ("vel_interior", "str_interior"):
+        return nxi * nyi * nzi
+    return None
+
+
+def cpu_baseline(case, steps):
+    """Reference CPU build (oracle/_ref, flang -O3 -ffast-math, serial) on the
+    same grid for a few steps; falls back to the oracle restatement."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refrun
+    sample = case.replace(ntfd=steps, stations=[])
+    if refrun.available(sample, fast=True):
+        res, wall, loop = refrun.run(sample, fast=True, keep_outputs=False, cores="0")
+        t = loop if loop else wall
+        return {"value": sample.cells * steps / t / 1e9, "unit": "Gcell-updates/s", "cores": 1,
+                "kind": "reference",
+                "sample": f"{case.name}: {case.nxt}x{case.nyt}x{case.nzt} cells, {steps} time steps, "
+                          f"reference Fortran built with flang -O3 -ffast-math, serial, pinned to core 0; "
+                          f"time = its own 'CPU TIME OF TIME LOOP' ({t:.2f} s)"}
+    import numpy as np
+    import oracle_lib as O
+    o = O.Oracle(sample, O.F32)
+    t0 = time.perf_counter()
+    o.step(steps, want_out=False)
+    t = time.perf_counter() - t0
+    return {"value": sample.cells * steps / t / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{case.name}: {steps} steps of the C++ oracle restatement, serial ({t:.2f} s)"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import numpy as np
+    from fd3d_tsn_amd import capi
+    from fd3d_tsn_amd import cases as C
+
+    case = workload(a.workload)
+    print(f"[bench] {case.name}: {case.nxt}x{case.nyt}x{case.nzt} = {case.cells/1e6:.1f}M cells, "
+          f"warmup {a.warmup}, steps {a.steps}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    fault, scal = (C.fault_fvw(case) if case.friction == "fvw" else (None, None))
+    s = capi.Solver(case, fault=fault, fvw_scalars=scal)
+    print(f"[bench] setup {time.perf_counter()-t0:.1f} s", file=sys.stderr, flush=True)
+    s.step(a.warmup, outputs=False)
+    s.sync()
+    s.set_profiling(True)
+    t0 = time.perf_counter()
+    s.step(a.steps, outputs=False)
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    kt = s.kernel_times()
+    s.set_profiling(False)
+    step_bytes, nbulk, npml = s.step_bytes()
+    s.close()
+
+    ms_per_step = elapsed * 1e3 / a.steps
+    value = case.cells * a.steps / elapsed / 1e9
+    # dominant kernel by total time
+    dom = max(kt.items(), key=lambda kv: kv[1][0])
+    name, (ms_tot, launches) = dom
+    avg_ms = ms_tot / launches
+    ncell = cells_of(case, name)
+    if name in BYTES_PER_CELL and ncell:
+        alg = BYTES_PER_CELL[name] * ncell
+    else:
+        alg = None
+    roof = {"bound": "hbm", "kernel": name,
+            "achieved": (alg / (avg_ms * 1e-3) / 1e9) if alg else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (alg / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if alg else None,
+            "traffic": None,
+            "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg}
+    kernels = {k: {"avg_ms": v[0] / v[1], "launches": v[1]} for k, v in sorted(kt.items())}
+    out = {
+        "metric": "Gcell-updates/s", "value": value, "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": case.name, "grid": [case.nxt, case.nyt, case.nzt], "dh": case.dh,
+                   "dt": case.dt, "nabc": case.nabc, "friction": case.friction,
+                   "fspace": case.fspace, "cells": case.cells},
+        "hbm_step": {"alg_bytes_per_step": step_bytes, "bulk_cells": nbulk, "pml_cells": npml,
+                     "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
+                     "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "roofline": roof,
+        "kernels": kernels,
+    }
+    if not a.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(case, a.cpu_steps)
+        except Exception as e:  # reported, not fatal: the GPU number stands
+            out["cpu_baseline"] = {"error": str(e)[:300]}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -84,6 +84,8 @@ def load(path: Optional[str] = None):
     L.fd3d_download_field.argtypes = [H, ctypes.c_int, _FP]
     L.fd3d_upload_field.argtypes = [H, ctypes.c_int, _FP]
     L.fd3d_set_profiling.argtypes = [H, ctypes.c_int]
+    L.fd3d_set_tuning.argtypes = [H] + [ctypes.c_int] * 5
+    L.fd3d_get_tuning.argtypes = [H, ctypes.POINTER(ctypes.c_int)]
     L.fd3d_kernel_times.argtypes = [H, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _DP,
                                     ctypes.POINTER(ctypes.c_long)]
     L.fd3d_step_bytes.restype = ctypes.c_double
@@ -237,6 +239,15 @@ class Solver:
     def set_field(self, name: str, arr: np.ndarray):
         a = _f32(arr)
         self._chk(self.L.fd3d_upload_field(self.h, FIELDS.index(name), _fp(a)))
+
+    # ---------------------------------------------------------------- tuning
+    def set_tuning(self, mode: int, tz_vel: int = 16, ny_vel: int = 64, tz_str: int = 16, ny_str: int = 64):
+        self._chk(self.L.fd3d_set_tuning(self.h, mode, tz_vel, ny_vel, tz_str, ny_str))
+
+    def get_tuning(self) -> Tuple[int, int, int, int, int]:
+        buf = (ctypes.c_int * 5)()
+        self._chk(self.L.fd3d_get_tuning(self.h, buf))
+        return tuple(buf)
 
     # ------------------------------------------------------------ profiling
     def set_profiling(self, on: bool):

@@ -75,6 +75,23 @@ Dims make_dims(const fd3d_config &c) {
   return g;
 }
 
+// Default launch shape: tall x-z tiles where the grid allows, and enough
+// y chunks for about 2048 blocks (8 per CU) on large grids.
+Tuning default_tuning(const Dims &g) {
+  Tuning t{};
+  t.mode = 1;
+  const int nx = g.nxt - 2 * g.nabc, nz = g.nzt - g.nfs - g.nabc, ny = g.nyt - 2 - g.nabc;
+  const int tz = nz >= 64 ? 16 : (nz >= 16 ? 8 : 4);
+  const long long tiles = (long long)((nx + 63) / 64) * ((nz + tz - 1) / tz);
+  long long gy = (2048 + tiles - 1) / tiles;
+  if (gy < 1) gy = 1;
+  int rows = ny > 0 ? (int)((ny + gy - 1) / gy) : 1;
+  if (rows < 8) rows = 8;
+  t.tz_vel = t.tz_str = tz;
+  t.ny_vel = t.ny_str = rows;
+  return t;
+}
+
 Coef make_coef(const fd3d_config &c) {
   Coef k{};
   k.dh = c.dh; k.dt = c.dt;
@@ -183,7 +200,7 @@ int dalloc(fd3d_handle *h, T **ptr, size_t n) {
   void *p = nullptr;
   hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
   if (e != hipSuccess) {
-    hipGetLastError();
+    (void)hipGetLastError();
     return fail(h, e == hipErrorOutOfMemory ? FD3D_E_NOMEM : FD3D_E_HIP,
                 std::string("hipMalloc: ") + hipGetErrorString(e));
   }
@@ -229,12 +246,12 @@ void record(fd3d_handle *h, KernelId id, bool start, hipEvent_t &ev) {
   (void)id;
   (void)start;
   if (h->pool.empty()) {
-    hipEventCreate(&ev);
+    (void)hipEventCreate(&ev);
   } else {
     ev = h->pool.back();
     h->pool.pop_back();
   }
-  hipEventRecord(ev, h->stream);
+  (void)hipEventRecord(ev, h->stream);
 }
 
 int launch_one(fd3d_handle *h, KernelId id) {
@@ -296,6 +313,7 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
   Problem &p = h->p;
   p.g = make_dims(*cfg);
   p.c = make_coef(*cfg);
+  p.tune = default_tuning(p.g);
   long long nsplit = 0, nprof = 0;
   if (build_regions(*cfg, p.g, p.pml, nsplit, nprof) != 0) {
     delete h;
@@ -591,6 +609,28 @@ int fd3d_upload_field(fd3d_handle *h, int which, const float *in) {
   return upload3(h, dst, in);
 }
 
+int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str) {
+  if (!h) return FD3D_E_INVALID;
+  auto ok_tz = [](int tz) { return tz == 4 || tz == 8 || tz == 16; };
+  if (mode != 0 && mode != 1) return fail(h, FD3D_E_INVALID, "tuning mode must be 0 or 1");
+  if (mode == 1 && (!ok_tz(tz_vel) || !ok_tz(tz_str) || ny_vel < 1 || ny_str < 1))
+    return fail(h, FD3D_E_INVALID, "tile height must be 4, 8 or 16 and rows >= 1");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->p.tune.mode = mode;
+  if (mode == 1) {
+    h->p.tune.tz_vel = tz_vel; h->p.tune.ny_vel = ny_vel;
+    h->p.tune.tz_str = tz_str; h->p.tune.ny_str = ny_str;
+  }
+  return 0;
+}
+
+int fd3d_get_tuning(const fd3d_handle *h, int *out5) {
+  if (!h || !out5) return FD3D_E_INVALID;
+  const Tuning &t = h->p.tune;
+  out5[0] = t.mode; out5[1] = t.tz_vel; out5[2] = t.ny_vel; out5[3] = t.tz_str; out5[4] = t.ny_str;
+  return 0;
+}
+
 int fd3d_set_profiling(fd3d_handle *h, int on) {
   if (!h) return FD3D_E_INVALID;
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -631,11 +671,11 @@ const char *fd3d_last_error(const fd3d_handle *h) { return h ? h->err.c_str() : 
 
 void fd3d_destroy(fd3d_handle *h) {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
-  for (auto &e : h->evs) { hipEventDestroy(e.a); hipEventDestroy(e.b); }
-  for (auto e : h->pool) hipEventDestroy(e);
-  for (void *p : h->allocs) hipFree(p);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto &e : h->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+  for (auto e : h->pool) (void)hipEventDestroy(e);
+  for (void *p : h->allocs) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 

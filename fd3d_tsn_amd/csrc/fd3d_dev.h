@@ -94,7 +94,16 @@ struct Outputs {
   int ring_steps, nsta;
 };
 
+// Launch shapes of the tiled interior sweeps (interior.hip); chosen per
+// grid size by the autotuner.  mode 0 = one thread per cell (kernels.hip).
+struct Tuning {
+  int mode;
+  int tz_vel, ny_vel;   // velocity: x-z tile height (4, 8, 16), rows per block along y
+  int tz_str, ny_str;   // stress
+};
+
 struct Problem {
+  Tuning tune;
   Dims g;
   Coef c;
   Fields f;
@@ -115,6 +124,8 @@ extern const char *kKernelNames[K_COUNT];
 
 // Launch one kernel of the step on a stream; returns hipSuccess or the launch error.
 hipError_t launch(KernelId id, const Problem &p, hipStream_t s);
+// Tiled interior sweep (velocity or stress), interior.hip.
+hipError_t launch_interior(bool stress, const Problem &p, hipStream_t s);
 // Fill helpers used by the prologue.
 hipError_t fill_f32(float *p, float v, long long n, hipStream_t s);
 

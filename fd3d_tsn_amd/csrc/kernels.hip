@@ -729,6 +729,7 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
   switch (id) {
     case K_VEL_INTERIOR:
       if (nyi <= 0) return hipSuccess;
+      if (p.tune.mode == 1) return launch_interior(false, p, s);
       hipLaunchKernelGGL(k_vel_interior, dim3(cdiv(ni, BX), cdiv(nyi, BY), nk), b2, 0, s, g, p.c, p.f);
       break;
     case K_VEL_FAULT:
@@ -747,6 +748,7 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
       break;
     case K_STR_INTERIOR:
       if (nyi <= 0) return hipSuccess;
+      if (p.tune.mode == 1) return launch_interior(true, p, s);
       hipLaunchKernelGGL(k_str_interior, dim3(cdiv(ni, BX), cdiv(nyi, BY), nk), b2, 0, s, g, p.c, p.f);
       break;
     case K_STR_FAULT:

@@ -142,6 +142,14 @@ int fd3d_download_final(fd3d_handle *h, float *out7);
 int fd3d_download_field(fd3d_handle *h, int which, float *out);
 int fd3d_upload_field(fd3d_handle *h, int which, const float *in);
 
+/* Launch shape of the tiled interior sweeps.  mode 0: one thread per cell;
+ * mode 1: a block owns a 64 x tz tile of the x-z plane and marches ny rows
+ * along y (tz in {4, 8, 16}, ny >= 1).  fd3d_create picks a default from
+ * the grid size; fd3d_tsn_amd/autotune.py measures and caches the best.
+ * Results do not depend on the shape (bit-identical). */
+int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str);
+int fd3d_get_tuning(const fd3d_handle *h, int *out5);
+
 /* Per-kernel timing with HIP events on the handle's stream (0 = off).
  * fd3d_kernel_times fills up to n entries of names/ms/launches for the
  * kernels launched since profiling was switched on; returns the count. */
