@@ -119,7 +119,7 @@ int build_regions(const fd3d_config &c, const Dims &g, Pml &pm, long long &nspli
   pm.nreg = (int)sp.size();
   nsplit = 0;
   nprof = 0;
-  int blk = 0;
+  int blk = 0, blkz = 0;
   for (int r = 0; r < pm.nreg; ++r) {
     PmlRegion &R = pm.r[r];
     const Spec &s = sp[r];
@@ -131,12 +131,17 @@ int build_regions(const fd3d_config &c, const Dims &g, Pml &pm, long long &nspli
     R.ztop = s.ztop; R.nzx = s.nzx; R.nzy = s.nzy;
     R.blk0 = blk;
     blk += (R.ncell + kPmlBlock - 1) / kPmlBlock;
+    R.nxyb = (R.ni * R.nj + kPmlBlock - 1) / kPmlBlock;
+    R.nzc = (R.nk + kPmlZChunk - 1) / kPmlZChunk;
+    R.blkz0 = blkz;
+    blkz += R.nxyb * R.nzc;
     R.split_off = nsplit;
     nsplit += (long long)kSplits * R.ncell;
     R.prof_off = (int)nprof;
     nprof += 4LL * (R.ni + 1) + 4LL * (R.nj + 1) + 4LL * (R.nk + 1);
   }
   pm.nblk = blk;
+  pm.nblkz = blkz;
   return 0;
 }
 
@@ -612,12 +617,12 @@ int fd3d_upload_field(fd3d_handle *h, int which, const float *in) {
 int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str) {
   if (!h) return FD3D_E_INVALID;
   auto ok_tz = [](int tz) { return tz == 4 || tz == 8 || tz == 16; };
-  if (mode != 0 && mode != 1) return fail(h, FD3D_E_INVALID, "tuning mode must be 0 or 1");
-  if (mode == 1 && (!ok_tz(tz_vel) || !ok_tz(tz_str) || ny_vel < 1 || ny_str < 1))
+  if (mode < 0 || mode > 3) return fail(h, FD3D_E_INVALID, "tuning mode must be 0..3");
+  if ((mode & 1) && (!ok_tz(tz_vel) || !ok_tz(tz_str) || ny_vel < 1 || ny_str < 1))
     return fail(h, FD3D_E_INVALID, "tile height must be 4, 8 or 16 and rows >= 1");
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->p.tune.mode = mode;
-  if (mode == 1) {
+  if (mode & 1) {
     h->p.tune.tz_vel = tz_vel; h->p.tune.ny_vel = ny_vel;
     h->p.tune.tz_str = tz_str; h->p.tune.ny_str = ny_str;
   }

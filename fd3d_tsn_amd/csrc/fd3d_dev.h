@@ -63,6 +63,7 @@ struct PmlRegion {
   int ni, nj, nk, ncell;
   int ztop, nzx, nzy;      // zero xz/yz splits at k2 = nk for j2 <= nzx / nzy
   int blk0;                // first block of this region in the fused PML launch
+  int blkz0, nxyb, nzc;    // z-march launch: first block, (i,j) blocks, z chunks
   long long split_off;     // offset of split 0 in the pool (floats)
   int prof_off;            // offset of the profile block in the profile pool
 };
@@ -70,9 +71,10 @@ struct PmlRegion {
 constexpr int kMaxRegions = 5;
 constexpr int kSplits = 24;
 constexpr int kPmlBlock = 256;
+constexpr int kPmlZChunk = 64;   // planes per thread in the z-march PML sweeps
 
 struct Pml {
-  int nreg, nblk;
+  int nreg, nblk, nblkz;
   PmlRegion r[kMaxRegions];
   float *splits;
   // per region 12 profile arrays: for p in (ox, oxS, oy, oyS, oz, ozS):
@@ -94,8 +96,10 @@ struct Outputs {
   int ring_steps, nsta;
 };
 
-// Launch shapes of the tiled interior sweeps (interior.hip); chosen per
-// grid size by the autotuner.  mode 0 = one thread per cell (kernels.hip).
+// Launch shapes of the interior and PML sweeps; chosen per grid size by
+// the autotuner.  mode bit 0: tiled interior sweeps (interior.hip) instead
+// of one thread per cell; bit 1: z-march PML sweeps (pml.hip) instead of
+// one thread per cell (kernels.hip).
 struct Tuning {
   int mode;
   int tz_vel, ny_vel;   // velocity: x-z tile height (4, 8, 16), rows per block along y
@@ -126,6 +130,8 @@ extern const char *kKernelNames[K_COUNT];
 hipError_t launch(KernelId id, const Problem &p, hipStream_t s);
 // Tiled interior sweep (velocity or stress), interior.hip.
 hipError_t launch_interior(bool stress, const Problem &p, hipStream_t s);
+// z-march PML sweep (velocity or stress), pml.hip.
+hipError_t launch_pml_zm(bool stress, const Problem &p, hipStream_t s);
 // Fill helpers used by the prologue.
 hipError_t fill_f32(float *p, float v, long long n, hipStream_t s);
 

@@ -729,7 +729,7 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
   switch (id) {
     case K_VEL_INTERIOR:
       if (nyi <= 0) return hipSuccess;
-      if (p.tune.mode == 1) return launch_interior(false, p, s);
+      if (p.tune.mode & 1) return launch_interior(false, p, s);
       hipLaunchKernelGGL(k_vel_interior, dim3(cdiv(ni, BX), cdiv(nyi, BY), nk), b2, 0, s, g, p.c, p.f);
       break;
     case K_VEL_FAULT:
@@ -738,6 +738,7 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
       break;
     case K_PML_VEL:
       if (p.pml.nblk == 0) return hipSuccess;
+      if (p.tune.mode & 2) return launch_pml_zm(false, p, s);
       hipLaunchKernelGGL(k_pml_vel, dim3(p.pml.nblk), dim3(kPmlBlock), 0, s, g, p.c, p.f, p.pml);
       break;
     case K_FUVW_UV:
@@ -748,7 +749,7 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
       break;
     case K_STR_INTERIOR:
       if (nyi <= 0) return hipSuccess;
-      if (p.tune.mode == 1) return launch_interior(true, p, s);
+      if (p.tune.mode & 1) return launch_interior(true, p, s);
       hipLaunchKernelGGL(k_str_interior, dim3(cdiv(ni, BX), cdiv(nyi, BY), nk), b2, 0, s, g, p.c, p.f);
       break;
     case K_STR_FAULT:
@@ -756,6 +757,7 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
       break;
     case K_PML_STR:
       if (p.pml.nblk == 0) return hipSuccess;
+      if (p.tune.mode & 2) return launch_pml_zm(true, p, s);
       hipLaunchKernelGGL(k_pml_str, dim3(p.pml.nblk), dim3(kPmlBlock), 0, s, g, p.c, p.f, p.pml);
       break;
     case K_FRES:

@@ -142,9 +142,10 @@ int fd3d_download_final(fd3d_handle *h, float *out7);
 int fd3d_download_field(fd3d_handle *h, int which, float *out);
 int fd3d_upload_field(fd3d_handle *h, int which, const float *in);
 
-/* Launch shape of the tiled interior sweeps.  mode 0: one thread per cell;
- * mode 1: a block owns a 64 x tz tile of the x-z plane and marches ny rows
- * along y (tz in {4, 8, 16}, ny >= 1).  fd3d_create picks a default from
+/* Launch shapes of the 3D sweeps.  mode bit 0: interior sweeps where a
+ * block owns a 64 x tz tile of the x-z plane and marches ny rows along y
+ * (tz in {4, 8, 16}, ny >= 1), else one thread per cell; mode bit 1: PML
+ * sweeps marching z in register windows, else one thread per cell.  fd3d_create picks a default from
  * the grid size; fd3d_tsn_amd/autotune.py measures and caches the best.
  * Results do not depend on the shape (bit-identical). */
 int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str);

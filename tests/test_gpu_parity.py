@@ -51,6 +51,7 @@ SW_CASES = [
     ("sw_free", dict(friction="sw", fspace=False)),
     ("sw_fs_odd", dict(friction="sw", fspace=True, nabc=5, n=(37, 13, 19), seed=5)),
     ("sw_free_wide", dict(friction="sw", fspace=False, nabc=4, n=(150, 9, 14), seed=7)),
+    ("sw_fs_tallz", dict(friction="sw", fspace=True, nabc=4, n=(24, 10, 150), seed=9)),
 ]
 
 
@@ -139,7 +140,7 @@ def test_fvw_against_mixed_and_fp64(name, kw):
     assert (rt_g < 1e4).sum() > 0, "no node ruptured"
 
 
-TUNINGS = [(0, 16, 64, 16, 64), (1, 4, 1, 4, 3), (1, 8, 5, 16, 2), (1, 16, 64, 8, 7), (1, 4, 1000, 4, 1000)]
+TUNINGS = [(0, 16, 64, 16, 64), (1, 4, 1, 4, 3), (3, 8, 5, 16, 2), (2, 16, 64, 8, 7), (3, 4, 1000, 4, 1000)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS, ids=[f"m{t[0]}_{t[1]}x{t[2]}_{t[3]}x{t[4]}" for t in TUNINGS])
@@ -153,7 +154,7 @@ def test_launch_shapes_bit_exact(tuning):
     with capi.Solver(case) as s:
         s.set_tuning(*tuning)
         got = s.get_tuning()
-        assert got[0] == tuning[0] and (tuning[0] == 0 or got == tuning)
+        assert got[0] == tuning[0] and (not (tuning[0] & 1) or got == tuning)
         rg = s.step(case.ntfd)
         fields = {n: s.field(n) for n in capi.FIELDS}
     for n in OUTS:
