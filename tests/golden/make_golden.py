@@ -1,0 +1,108 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures of tests/golden/ from the reference build.
+
+Runs oracle/_ref/fd3d_ref_* (the unmodified FD3D_TSN Fortran sources built
+with flang by ``make -C oracle ref`` and driven by oracle/ref_harness.f90)
+on small seeded problems and stores what the reference wrote:
+
+* slip weakening (FP32 build): SHA-256 of every per-step output slab of
+  result/sliprateX/Z.res and shearstressX/Z.res, the last slab of each,
+  the station samples and the final 2D arrays;
+* rate-and-state (FP64 build): the same final arrays and stations, plus the
+  time histories of every output at a sub-sampled set of fault nodes.
+
+Usage:  python tests/golden/make_golden.py   (needs oracle/_ref built)
+"""
+import dataclasses
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from fd3d_tsn_amd import cases as C  # noqa: E402
+import refrun  # noqa: E402
+
+
+def spread(case):
+    nx, ny, nz, nfs = case.nxt, case.nyt, case.nzt, case.nfs
+    st = [(nx // 2, ny // 2, nz // 2), (nx // 2 + 1, ny - 1, nz // 2), (nx // 2 + 2, ny, nz // 2 + 1),
+          (2, ny - 2, nz // 2), (nx - 1, ny - 3, nz // 2 - 1), (nx // 2, 2, nz // 2), (nx // 2, ny // 2, 2),
+          (nx // 2 - 1, ny, nz - nfs), (2, 2, 2), (nx - 2, ny, nz - 1)]
+    return case.replace(stations=st)
+
+
+def golden_cases():
+    return {
+        "sw_fs_tiny": spread(C.tiny("sw", True, ntfd=120, seed=11)),
+        "sw_free_tiny": spread(C.tiny("sw", False, ntfd=120, seed=12)),
+        "fvw_free_tiny": spread(C.tiny("fvw", False, ntfd=150, seed=13)),
+        "fvw_fs_tiny": spread(C.tiny("fvw", True, ntfd=150, seed=14)),
+        "plain_sw_128x64x64": spread(C.c1_plain_sw()),
+    }
+
+
+OUTS = ("sliprateX", "sliprateZ", "shearstressX", "shearstressZ")
+FINALS = ("ruptime", "rise", "sliptime", "slipX", "slipZ", "stressdropX", "stressdropZ")
+
+
+def slab_hash(a32: np.ndarray) -> bytes:
+    """SHA-256 of one slab in the byte order of the .res stream (x fastest)."""
+    return hashlib.sha256(np.ascontiguousarray(a32.T, dtype=np.float32).tobytes()).digest()
+
+
+def sub_nodes(nxT, nzT):
+    return np.arange(1, nxT, 3), np.arange(1, nzT, 3)
+
+
+def main():
+    manifest = {}
+    for name, case in golden_cases().items():
+        print(f"[golden] {name}: {case.nxt}x{case.nyt}x{case.nzt}, {case.ntfd} steps", flush=True)
+        res, wall, loop = refrun.run(case)
+        fvw = case.friction == "fvw"
+        data = {}
+        for n in FINALS:
+            data[f"final_{n}"] = res[n].astype(np.float64 if fvw else np.float32)
+        data["seis"] = res["seis"].astype(np.float64 if fvw else np.float32)
+        if not fvw:
+            hashes = np.zeros((case.ntfd, len(OUTS), 32), dtype=np.uint8)
+            for o, n in enumerate(OUTS):
+                a = res[n].astype(np.float32)
+                for t in range(case.ntfd):
+                    hashes[t, o] = np.frombuffer(slab_hash(a[t]), dtype=np.uint8)
+                data[f"last_{n}"] = a[-1]
+            data["slab_sha256"] = hashes
+        else:
+            xi, zk = sub_nodes(*case.fault_shape)
+            for n in OUTS + ("psi",):
+                data[f"hist_{n}"] = res[n][:, xi][:, :, zk]
+            data["hist_x"] = xi
+            data["hist_z"] = zk
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **data)
+        cfg = dataclasses.asdict(case)
+        cfg["stations"] = [list(s) for s in case.stations]
+        manifest[name] = {
+            "case": cfg,
+            "reference_build": os.path.basename(refrun.variant(case)),
+            "precision": "float64 (-fdefault-real-8)" if fvw else "float32",
+            "flags": "flang -cpp -O2 -ffp-contract=off" + (" -DFVW -DTPV104" if fvw else "") +
+                     (" -DFSPACE" if case.fspace else ""),
+            "file": os.path.basename(path),
+            "sha256": hashlib.sha256(open(path, "rb").read()).hexdigest(),
+            "reference_loop_seconds": loop,
+        }
+    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
+    print("[golden] wrote", ", ".join(manifest))
+
+
+if __name__ == "__main__":
+    main()
