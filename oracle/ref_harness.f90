@@ -6,12 +6,20 @@
 ! The reference's own setup routines cannot express seeded random stress
 ! or friction fields (forwardspecialTPV5 only knows rectangular patches,
 ! readinversionresult forces dyn_xz = coh = 0, fd3d_init.f90:830-835), so
-! this program reads the fault fields from 'fault_in.bin' after
+! by default this program reads the fault fields from 'fault_in.bin' after
 ! fd3d_init()/inversion_init() (dynamicsolver.f90:35-36) and then calls
 ! fd3d() exactly as dynamicsolver.f90:62-66 does.  After the loop it
 ! writes the 2D result arrays as one raw stream, 'result/final.bin'.
 !
-! fault_in.bin layout (Fortran order, default REAL kind of the build):
+! Optional file 'harness_setup.txt' (first word):
+!   file       read fault_in.bin (default)
+!   tpv5       call the reference's forwardspecialTPV5   (slip weakening builds)
+!   tpv104     call the reference's forwardspecialTPV104 (rate-and-state builds)
+!   inversion  call the reference's readinversionresult (forwardmodel.dat)
+! followed optionally by the word 'dump': write the fault arrays the setup
+! produced to 'setup_dump.bin' and stop before the time loop.
+!
+! fault_in.bin / setup_dump.bin layout (Fortran order, default REAL kind):
 !   slip weakening : striniX striniZ peak_xz dyn_xz Dc coh  (nxt x nzt each)
 !   FVW            : Dc striniX striniZ a b psi vw aX bX psiX vwX
 !                    then scalars Sn uini wini f0 v0 fw hx0 hz0 perturb
@@ -25,19 +33,58 @@
       USE SlipRates_com
       USE inversion_com
       IMPLICIT NONE
-      integer :: u
+      integer :: u, ios
+      character(len=32) :: setup, action
+      logical :: have
+
+      setup = 'file'
+      action = ''
+      inquire(file='harness_setup.txt', exist=have)
+      if (have) then
+        open(newunit=u, file='harness_setup.txt', status='old')
+        read(u, *, iostat=ios) setup, action
+        close(u)
+      endif
 
       call fd3d_init()
       call inversion_init()
 
-      open(newunit=u, file='fault_in.bin', form='unformatted', access='stream', status='old')
+      select case (trim(setup))
+      case ('file')
+        open(newunit=u, file='fault_in.bin', form='unformatted', access='stream', status='old')
 #if defined FVW
-      read(u) Dc, striniX, striniZ, a, b, psi, vw, aX, bX, psiX, vwX
-      read(u) Sn, uini, wini, f0, v0, fw, hx0, hz0, perturb, RR2, TT2, strinixI
+        read(u) Dc, striniX, striniZ, a, b, psi, vw, aX, bX, psiX, vwX
+        read(u) Sn, uini, wini, f0, v0, fw, hx0, hz0, perturb, RR2, TT2, strinixI
 #else
-      read(u) striniX, striniZ, peak_xz, dyn_xz, Dc, coh
+        read(u) striniX, striniZ, peak_xz, dyn_xz, Dc, coh
 #endif
-      close(u)
+        close(u)
+#if defined FVW
+      case ('tpv104')
+        call forwardspecialTPV104()
+#else
+      case ('tpv5')
+        call forwardspecialTPV5()
+#endif
+      case ('inversion')
+        call readinversionresult()
+      case default
+        print *, 'unknown setup ', trim(setup)
+        stop 1
+      end select
+
+      if (trim(action) == 'dump') then
+        open(newunit=u, file='setup_dump.bin', form='unformatted', access='stream', status='replace')
+#if defined FVW
+        write(u) Dc, striniX, striniZ, a, b, psi, vw, aX, bX, psiX, vwX
+        write(u) Sn, uini, wini, f0, v0, fw, hx0, hz0, perturb, RR2, TT2, strinixI
+#else
+        write(u) striniX, striniZ, peak_xz, dyn_xz, Dc, coh
+#endif
+        write(u) lam1, mu1, d1
+        close(u)
+        stop
+      endif
 
       ioutput = 1
       call fd3d()

@@ -61,8 +61,77 @@ def sub_nodes(nxT, nzT):
     return np.arange(1, nxT, 3), np.arange(1, nzT, 3)
 
 
+DRIVER_FILES = ["sliprateX.res", "sliprateZ.res", "shearstressX.res", "shearstressZ.res", "ruptime.res",
+                "risetime.res", "slipX.res", "slipZ.res", "stressdropX.res", "stressdropZ.res", "contour.res",
+                "czone.res", "rvel.res", "vmodel.inp", "friction.inp"]
+
+
+def driver_cases():
+    """Small problems whose whole result/ directory the driver must reproduce."""
+    return {
+        "files_sw_fs": spread(C.tiny("sw", True, ntfd=90, seed=21)).replace(stations=[(20, 10, 12), (26, 22, 15)]),
+        "files_sw_free": spread(C.tiny("sw", False, ntfd=90, seed=22)).replace(stations=[(21, 9, 20)]),
+    }
+
+
+def driver_fixtures(manifest):
+    import shutil
+    import tempfile
+    for name, case in driver_cases().items():
+        print(f"[golden] {name}: reference result/ files", flush=True)
+        wd = tempfile.mkdtemp(prefix="fd3dgold_")
+        refrun.run(case, workdir=wd)
+        files = DRIVER_FILES + [f"stan{s}.txt" for s in range(len(case.stations))]
+        digests = {f: hashlib.sha256(open(os.path.join(wd, "result", f), "rb").read()).hexdigest() for f in files}
+        inputs = os.path.join(HERE, name)
+        os.makedirs(inputs, exist_ok=True)
+        for f in ("inputfd3d.dat", "crustal.dat", "fault_in.bin"):
+            shutil.copy(os.path.join(wd, f), os.path.join(inputs, f))
+        shutil.rmtree(wd)
+        cfg = dataclasses.asdict(case)
+        cfg["stations"] = [list(s) for s in case.stations]
+        manifest[name] = {"case": cfg, "reference_build": os.path.basename(refrun.variant(case)),
+                          "inputs": name, "result_sha256": digests}
+
+
+EXAMPLE_RUNS = {
+    # fixture: (example, reference variant, harness setup, steps)
+    "files_tpv5_200": ("TPV5", "fd3d_ref_sw_free", "tpv5", 200),
+    "files_amatrice_200": ("Amatrice2016", "fd3d_ref_sw_free", "inversion", 200),
+}
+
+
+def example_fixtures(manifest):
+    """The reference's own example inputs (tests/golden/examples/), time loop
+    cut to a few hundred steps: the result/ files the driver must reproduce."""
+    import shutil
+    import subprocess
+    import tempfile
+    for name, (example, variant, setup, steps) in EXAMPLE_RUNS.items():
+        print(f"[golden] {name}: {example} ({setup}), {steps} steps", flush=True)
+        wd = tempfile.mkdtemp(prefix="fd3dgold_")
+        src = os.path.join(HERE, "examples", example)
+        for f in os.listdir(src):
+            shutil.copy(os.path.join(src, f), os.path.join(wd, f))
+        lines = open(os.path.join(wd, "inputfd3d.dat")).read().split("\n")
+        lines[2] = f"{steps}"
+        open(os.path.join(wd, "inputfd3d.dat"), "w").write("\n".join(lines))
+        open(os.path.join(wd, "harness_setup.txt"), "w").write(setup + "\n")
+        os.makedirs(os.path.join(wd, "result"))
+        p = subprocess.run([os.path.join(refrun.REF_DIR, variant)], cwd=wd, capture_output=True, text=True)
+        assert p.returncode == 0, p.stdout[-2000:]
+        nsta = int(lines[7].split()[0])
+        files = DRIVER_FILES + [f"stan{s}.txt" for s in range(nsta)]
+        digests = {f: hashlib.sha256(open(os.path.join(wd, "result", f), "rb").read()).hexdigest() for f in files}
+        manifest[name] = {"example": example, "setup": setup, "steps": steps, "reference_build": variant,
+                          "result_sha256": digests}
+        shutil.rmtree(wd)
+
+
 def main():
     manifest = {}
+    example_fixtures(manifest)
+    driver_fixtures(manifest)
     for name, case in golden_cases().items():
         print(f"[golden] {name}: {case.nxt}x{case.nyt}x{case.nzt}, {case.ntfd} steps", flush=True)
         res, wall, loop = refrun.run(case)
