@@ -65,6 +65,21 @@ def cells_of(case, kernel):
     return None
 
 
+def profiled_traffic(workload_name, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    plain-copy calibration recorded in the same file); None when absent."""
+    path = os.path.join(ROOT, "profiles", f"{workload_name}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel:
+                return float(w[2]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(case, steps):
     """Reference CPU build (oracle/_ref, flang -O3 -ffast-math, serial) on the
     same grid for a few steps; falls back to the oracle restatement."""
@@ -100,6 +115,14 @@ def main():
     from fd3d_tsn_amd import cases as C
 
     case = workload(a.workload)
+    # CPU baseline first, before this process initialises the GPU
+    cpu = None
+    if not a.no_cpu_baseline:
+        print(f"[bench] cpu baseline: {a.cpu_steps} steps of the reference build", file=sys.stderr, flush=True)
+        try:
+            cpu = cpu_baseline(case, a.cpu_steps)
+        except Exception as e:  # reported, not fatal: the GPU number stands
+            cpu = {"error": str(e)[:300]}
     print(f"[bench] {case.name}: {case.nxt}x{case.nyt}x{case.nzt} = {case.cells/1e6:.1f}M cells, "
           f"warmup {a.warmup}, steps {a.steps}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
@@ -129,12 +152,15 @@ def main():
         alg = BYTES_PER_CELL[name] * ncell
     else:
         alg = None
+    traffic, traffic_src = profiled_traffic(case.name, name)
     roof = {"bound": "hbm", "kernel": name,
             "achieved": (alg / (avg_ms * 1e-3) / 1e9) if alg else None,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (alg / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if alg else None,
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg}
+    if traffic_src:
+        roof["traffic_source"] = traffic_src
     kernels = {k: {"avg_ms": v[0] / v[1], "launches": v[1]} for k, v in sorted(kt.items())}
     out = {
         "metric": "Gcell-updates/s", "value": value, "unit": "Gcell-updates/s", "n_gpus": 1,
@@ -149,11 +175,8 @@ def main():
         "roofline": roof,
         "kernels": kernels,
     }
-    if not a.no_cpu_baseline:
-        try:
-            out["cpu_baseline"] = cpu_baseline(case, a.cpu_steps)
-        except Exception as e:  # reported, not fatal: the GPU number stands
-            out["cpu_baseline"] = {"error": str(e)[:300]}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 

@@ -10,6 +10,7 @@ the station text files (:555-557) and the final 2D arrays.
 from __future__ import annotations
 
 import os
+import resource
 import subprocess
 import tempfile
 import time
@@ -88,6 +89,13 @@ def read_outputs(case: C.Case, wd: str) -> Dict[str, np.ndarray]:
     return res
 
 
+def _big_stack():
+    """fd3d() keeps several (nxt, nzt) automatic arrays on the stack
+    (fd3d_theo.f90:36,43,47): lift the soft stack limit to the hard one."""
+    soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+    resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+
 def run(case: C.Case, fault=None, fvw_scalars=None, fast: bool = False, timeout: float = 3600,
         workdir: Optional[str] = None, keep_outputs: bool = True, cores: Optional[str] = None):
     """Run the reference; returns (outputs, wall seconds, loop CPU seconds)."""
@@ -100,7 +108,7 @@ def run(case: C.Case, fault=None, fvw_scalars=None, fast: bool = False, timeout:
     if cores is not None:
         cmd = ["taskset", "-c", cores, exe]
     t0 = time.perf_counter()
-    p = subprocess.run(cmd, cwd=wd, capture_output=True, text=True, timeout=timeout)
+    p = subprocess.run(cmd, cwd=wd, capture_output=True, text=True, timeout=timeout, preexec_fn=_big_stack)
     wall = time.perf_counter() - t0
     if p.returncode != 0:
         raise RuntimeError(f"reference failed rc={p.returncode}\n{p.stdout[-2000:]}\n{p.stderr[-2000:]}")
