@@ -40,7 +40,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="large_sw", choices=["large_sw", "plain_sw", "fvw"])
+    ap.add_argument("--workload", default="large_sw", choices=["large_sw", "plain_sw", "fvw", "batched"])
+    ap.add_argument("--batch-mode", default="staged", choices=["persistent", "staged"])
+    ap.add_argument("--batch-tz", type=int, default=16)
+    ap.add_argument("--batch-n", type=int, default=64, help="domains of the batched workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="time steps of the reference CPU build on the same grid")
@@ -104,12 +107,78 @@ def cpu_baseline(case, steps):
             "sample": f"{case.name}: {steps} steps of the C++ oracle restatement, serial ({t:.2f} s)"}
 
 
+def bench_batched(a):
+    """Batched uneven domains (64 slip-weakening ruptures, sides 96..320,
+    1500 steps each): time per step of the batch against the sum of the
+    per-step times of each domain stepped alone (back-to-back plain
+    solves); forward ruptures/s follow from the 1500-step length."""
+    from fd3d_tsn_amd import capi
+    from fd3d_tsn_amd import cases as C
+    cases = C.c4_batched(a.batch_n)
+    nsteps_run = cases[0].ntfd
+    t0 = time.perf_counter()
+    solvers = []
+    setup_s = []
+    for c in cases:
+        t1 = time.perf_counter()
+        solvers.append(capi.Solver(c))
+        setup_s.append(time.perf_counter() - t1)
+    print(f"[bench] batched: {len(cases)} domains, {sum(c.cells for c in cases)/1e6:.0f}M cells, "
+          f"setup {time.perf_counter()-t0:.1f} s", file=sys.stderr, flush=True)
+    plain = []
+    for s in solvers:
+        s.step(a.warmup, outputs=False)
+        s.sync()
+        t1 = time.perf_counter()
+        s.step(a.steps, outputs=False)
+        s.sync()
+        plain.append((time.perf_counter() - t1) / a.steps)
+    for s in solvers:
+        s.init()
+    mode = capi.PERSISTENT if a.batch_mode == "persistent" else capi.STAGED
+    b = capi.Batch(len(solvers), tz=a.batch_tz, mode=mode)
+    for s in solvers:
+        b.add(s)
+    b.step(a.warmup)
+    b.sync()
+    b.elapsed()
+    t1 = time.perf_counter()
+    b.step(a.steps)
+    b.sync()
+    wall = (time.perf_counter() - t1) / a.steps
+    gpu_ms, n = b.elapsed()
+    info = b.info()
+    b.close()
+    for s in solvers:
+        s.close()
+    cells = sum(c.cells for c in cases)
+    t_plain = sum(plain)
+    value = len(cases) / (nsteps_run * wall)
+    out = {"metric": "forward ruptures/s (batched uneven domains)", "value": value,
+           "unit": "ruptures/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"batched_{len(cases)}_domains_96-320", "domains": len(cases),
+                      "cells": cells, "steps_per_rupture": nsteps_run, "mode": a.batch_mode,
+                      "tz": a.batch_tz},
+           "batch": {"gpu_ms_per_step": gpu_ms / max(n, 1), "Gcell_updates_per_s": cells / wall / 1e9,
+                     "info": info},
+           "plain_back_to_back": {"ms_per_step_sum": t_plain * 1e3,
+                                  "ruptures_per_s": len(cases) / (nsteps_run * t_plain),
+                                  "Gcell_updates_per_s": cells / t_plain / 1e9,
+                                  "setup_s_per_rupture": sum(setup_s) / len(setup_s)},
+           "speedup_vs_plain": t_plain / wall}
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.workload == "batched":
+        return bench_batched(a)
     import numpy as np
     from fd3d_tsn_amd import capi
     from fd3d_tsn_amd import cases as C

@@ -93,6 +93,21 @@ def load(path: Optional[str] = None):
     L.fd3d_last_error.restype = ctypes.c_char_p
     L.fd3d_last_error.argtypes = [H]
     L.fd3d_destroy.argtypes = [H]
+    L.fd3d_read_outputs.argtypes = [H, ctypes.c_int, _FP, _FP]
+    L.fd3d_device_memory.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+    B = ctypes.c_void_p
+    L.fd3d_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(B)]
+    L.fd3d_batch_add.argtypes = [B, H]
+    L.fd3d_batch_remove.argtypes = [B, H]
+    L.fd3d_batch_size.argtypes = [B]
+    L.fd3d_batch_step.argtypes = [B, ctypes.c_int]
+    L.fd3d_batch_sync.argtypes = [B]
+    L.fd3d_batch_info.argtypes = [B, ctypes.POINTER(ctypes.c_int)]
+    L.fd3d_batch_elapsed.restype = ctypes.c_double
+    L.fd3d_batch_elapsed.argtypes = [B, ctypes.POINTER(ctypes.c_int)]
+    L.fd3d_batch_last_error.restype = ctypes.c_char_p
+    L.fd3d_batch_last_error.argtypes = [B]
+    L.fd3d_batch_destroy.argtypes = [B]
     if path is None:
         _lib = L
     return L
@@ -138,6 +153,15 @@ def make_config(case: C.Case, fvw_scalars: Optional[Dict[str, float]] = None, ne
 def footprint(case: C.Case) -> int:
     cfg, _sta = make_config(case)
     return int(load().fd3d_footprint(ctypes.byref(cfg)))
+
+
+def device_memory() -> Tuple[int, int]:
+    """(free, total) bytes of device memory."""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    rc = load().fd3d_device_memory(ctypes.byref(f), ctypes.byref(t))
+    if rc != 0:
+        raise FD3DError(rc, "hipMemGetInfo failed")
+    return int(f.value), int(t.value)
 
 
 class Solver:
@@ -220,6 +244,22 @@ class Solver:
         res["seis"] = seis[:, :nsta, :]
         return res
 
+    def read_outputs(self, nsteps: int) -> Dict[str, np.ndarray]:
+        """Slabs and stations of the last nsteps steps from the device ring
+        (for problems advanced by a Batch)."""
+        case = self.case
+        nxT, nzT = case.fault_shape
+        nsta = len(case.stations)
+        out = np.empty((nsteps, NOUT, nzT, nxT), dtype=np.float32)
+        seis = np.empty((nsteps, max(nsta, 1), 3), dtype=np.float32)
+        self._chk(self.L.fd3d_read_outputs(self.h, nsteps, _fp(out), _fp(seis) if nsta else None))
+        res = {name: out[:, o].transpose(0, 2, 1) for o, name in enumerate(OUT_NAMES)}
+        res["seis"] = seis[:, :nsta, :]
+        return res
+
+    def steps_done(self) -> int:
+        return int(self.L.fd3d_steps_done(self.h))
+
     def sync(self):
         self._chk(self.L.fd3d_sync(self.h))
 
@@ -268,6 +308,74 @@ class Solver:
         p = ctypes.c_long()
         v = self.L.fd3d_step_bytes(self.h, ctypes.byref(b), ctypes.byref(p))
         return v, b.value, p.value
+
+
+PERSISTENT, STAGED = 0, 1
+
+
+class Batch:
+    """Many resident problems advanced together (fd3d_batch_*): one
+    persistent launch per time step (or one launch per stage), replayed
+    from a captured HIP graph.  Members are Solver objects; their outputs
+    are read back with Solver.read_outputs / Solver.final."""
+
+    def __init__(self, max_problems: int, tz: int = 16, mode: int = 1):
+        self.L = load()
+        self.b = ctypes.c_void_p()
+        rc = self.L.fd3d_batch_create(max_problems, tz, mode, ctypes.byref(self.b))
+        if rc != 0:
+            self.b = None
+            raise FD3DError(rc, "fd3d_batch_create failed")
+        self.members = []
+
+    def _chk(self, rc: int):
+        if rc != 0:
+            raise FD3DError(rc, self.L.fd3d_batch_last_error(self.b).decode() if self.b else "")
+
+    def add(self, s: Solver):
+        self._chk(self.L.fd3d_batch_add(self.b, s.h))
+        self.members.append(s)
+
+    def remove(self, s: Solver):
+        self._chk(self.L.fd3d_batch_remove(self.b, s.h))
+        self.members.remove(s)
+
+    def __len__(self):
+        return int(self.L.fd3d_batch_size(self.b))
+
+    def step(self, nsteps: int):
+        self._chk(self.L.fd3d_batch_step(self.b, nsteps))
+
+    def sync(self):
+        self._chk(self.L.fd3d_batch_sync(self.b))
+
+    def info(self) -> Dict[str, int]:
+        buf = (ctypes.c_int * 4)()
+        self._chk(self.L.fd3d_batch_info(self.b, buf))
+        return {"workers": buf[0], "items": buf[1], "segments": buf[2], "launches": buf[3]}
+
+    def elapsed(self) -> Tuple[float, int]:
+        n = ctypes.c_int()
+        ms = self.L.fd3d_batch_elapsed(self.b, ctypes.byref(n))
+        return ms, n.value
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.L.fd3d_batch_destroy(self.b)
+            self.b = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def gpu_available() -> bool:

@@ -14,35 +14,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/fd3d_tsn.h"
-#include "fd3d_dev.h"
+#include "handle.h"
 
 using namespace fd3d;
-
-struct fd3d_handle {
-  fd3d_config cfg{};
-  std::vector<int> sta;
-  Problem p{};
-  hipStream_t stream = nullptr;
-  std::vector<void *> allocs;
-  size_t bytes = 0;
-  bool medium_ok = false, fault_ok = false, inited = false;
-  int steps = 0;
-  std::string err;
-  // host copies of the fault inputs (staggering is done on the host)
-  std::vector<float> h_striniX, h_striniZ, h_peak, h_dyn, h_Dc, h_coh;
-  std::vector<double> h_a, h_b, h_psi, h_vw, h_aX, h_bX, h_psiX, h_vwX;
-  int *d_sta = nullptr;
-  // profiling
-  bool prof = false;
-  struct Ev { int id; hipEvent_t a, b; };
-  std::vector<Ev> evs;
-  std::vector<hipEvent_t> pool;
-  double kms[K_COUNT] = {0};
-  long kcount[K_COUNT] = {0};
-  // step sequence
-  std::vector<KernelId> seq;
-};
 
 namespace {
 
@@ -284,10 +258,22 @@ void harvest(fd3d_handle *h) {
   h->evs.clear();
 }
 
-int check_ctl(fd3d_handle *h) {
+// Wait for the work queued on h: its own stream, and the stream of the
+// batch that advances it.
+hipError_t owner_sync(fd3d_handle *h) {
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && h->batch) e = hipStreamSynchronize(fd3d_batch_stream_of(h->batch));
+  return e;
+}
+
+int check_ctl(fd3d_handle *h) { return fd3d_handle_check_ctl(h, h->stream); }
+
+}  // namespace
+
+int fd3d_handle_check_ctl(fd3d_handle *h, hipStream_t s) {
   Ctl c{};
-  HIPCHK(h, hipMemcpyAsync(&c, h->p.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipMemcpyAsync(&c, h->p.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+  HIPCHK(h, hipStreamSynchronize(s));
   if (c.newton_capped > 0)
     return fail(h, FD3D_E_NEWTON, "rate-and-state Newton solve hit the iteration cap at " +
                                       std::to_string(c.newton_capped) + " node-steps");
@@ -295,13 +281,19 @@ int check_ctl(fd3d_handle *h) {
   return 0;
 }
 
-}  // namespace
-
 extern "C" {
 
 size_t fd3d_footprint(const fd3d_config *cfg) {
   if (validate(cfg) != 0) return 0;
   return footprint(*cfg);
+}
+
+int fd3d_device_memory(size_t *free_bytes, size_t *total_bytes) {
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) return FD3D_E_HIP;
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return 0;
 }
 
 int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
@@ -392,7 +384,7 @@ static int upload3(fd3d_handle *h, float *dst, const float *src) {
 
 static int download3(fd3d_handle *h, float *dst, const float *src) {
   const Dims &g = h->p.g;
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, owner_sync(h));
   HIPCHK(h, hipMemcpy2D(dst, (size_t)g.nxt * sizeof(float), src + g.off0, (size_t)g.px * sizeof(float),
                         (size_t)g.nxt * sizeof(float), (size_t)g.nyt * g.nzt, hipMemcpyDeviceToHost));
   return 0;
@@ -456,6 +448,7 @@ int fd3d_upload_fault_fvw(fd3d_handle *h, const double *Dc, const double *strini
 int fd3d_init_state(fd3d_handle *h) {
   if (!h) return FD3D_E_INVALID;
   if (!h->medium_ok || !h->fault_ok) return fail(h, FD3D_E_STATE, "upload the medium and the fault first");
+  if (h->batch) return fail(h, FD3D_E_STATE, "remove the problem from its batch first");
   Problem &p = h->p;
   const Dims &g = p.g;
   const size_t n3 = (size_t)g.n3;
@@ -528,6 +521,7 @@ int fd3d_init_state(fd3d_handle *h) {
 int fd3d_step(fd3d_handle *h, int nsteps, float *out, float *seis) {
   if (!h) return FD3D_E_INVALID;
   if (!h->inited) return fail(h, FD3D_E_STATE, "fd3d_init_state first");
+  if (h->batch) return fail(h, FD3D_E_STATE, "the problem is advanced by a batch (fd3d_batch_step)");
   if (nsteps < 0) return fail(h, FD3D_E_INVALID, "negative step count");
   const int ring = h->cfg.ring_steps;
   const size_t slab = (size_t)h->p.g.nxT * h->p.g.nzT;
@@ -560,15 +554,40 @@ int fd3d_step(fd3d_handle *h, int nsteps, float *out, float *seis) {
 
 int fd3d_sync(fd3d_handle *h) {
   if (!h) return FD3D_E_INVALID;
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, owner_sync(h));
   return check_ctl(h);
 }
 
 int fd3d_steps_done(const fd3d_handle *h) { return h ? h->steps : 0; }
 
+int fd3d_read_outputs(fd3d_handle *h, int nsteps, float *out, float *seis) {
+  if (!h) return FD3D_E_INVALID;
+  const int ring = h->cfg.ring_steps;
+  if (nsteps < 0 || nsteps > ring || nsteps > h->steps)
+    return fail(h, FD3D_E_INVALID, "nsteps must be within the steps taken and the output ring");
+  HIPCHK(h, owner_sync(h));
+  const size_t slab = (size_t)h->p.g.nxT * h->p.g.nzT;
+  const int nsta = h->cfg.nstations;
+  // step s (1-based) sits in slot (s-1) % ring
+  int done = 0;
+  while (done < nsteps) {
+    const int first = h->steps - nsteps + done + 1;
+    const int slot = (first - 1) % ring;
+    const int n = std::min(nsteps - done, ring - slot);
+    if (out)
+      HIPCHK(h, hipMemcpy(out + (size_t)done * FD3D_NOUT * slab, h->p.out.ring + (size_t)slot * FD3D_NOUT * slab,
+                          (size_t)n * FD3D_NOUT * slab * sizeof(float), hipMemcpyDeviceToHost));
+    if (seis && nsta > 0)
+      HIPCHK(h, hipMemcpy(seis + (size_t)done * nsta * 3, h->p.out.seis + (size_t)slot * nsta * 3,
+                          (size_t)n * nsta * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    done += n;
+  }
+  return check_ctl(h);
+}
+
 int fd3d_download_final(fd3d_handle *h, float *out7) {
   if (!h || !out7) return fail(h, FD3D_E_INVALID, "null argument");
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, owner_sync(h));
   const size_t n2 = (size_t)h->cfg.nxt * h->cfg.nzt;
   std::vector<float> t0x(n2), t0z(n2), rise(n2), schx(n2), schz(n2);
   Fault &fl = h->p.fl;
@@ -610,7 +629,7 @@ int fd3d_upload_field(fd3d_handle *h, int which, const float *in) {
   if (!h || !in) return fail(h, FD3D_E_INVALID, "null argument");
   float *dst = field_ptr(h, which);
   if (!dst) return fail(h, FD3D_E_INVALID, "unknown field");
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, owner_sync(h));
   return upload3(h, dst, in);
 }
 
@@ -620,6 +639,7 @@ int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str
   if (mode < 0 || mode > 3) return fail(h, FD3D_E_INVALID, "tuning mode must be 0..3");
   if ((mode & 1) && (!ok_tz(tz_vel) || !ok_tz(tz_str) || ny_vel < 1 || ny_str < 1))
     return fail(h, FD3D_E_INVALID, "tile height must be 4, 8 or 16 and rows >= 1");
+  if (h->batch) return fail(h, FD3D_E_STATE, "remove the problem from its batch first");
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->p.tune.mode = mode;
   if (mode & 1) {
@@ -676,6 +696,7 @@ const char *fd3d_last_error(const fd3d_handle *h) { return h ? h->err.c_str() : 
 
 void fd3d_destroy(fd3d_handle *h) {
   if (!h) return;
+  if (h->batch) (void)fd3d_batch_remove(h->batch, h);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (auto &e : h->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto e : h->pool) (void)hipEventDestroy(e);

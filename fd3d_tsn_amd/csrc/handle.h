@@ -1,0 +1,47 @@
+// Host-side state of one problem (fd3d_handle), shared by the single-problem
+// C ABI (capi.hip) and the multi-domain batch engine (batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fd3d_tsn.h"
+#include "fd3d_dev.h"
+
+using namespace fd3d;
+
+struct fd3d_handle {
+  fd3d_config cfg{};
+  std::vector<int> sta;
+  Problem p{};
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  bool medium_ok = false, fault_ok = false, inited = false;
+  int steps = 0;
+  std::string err;
+  // host copies of the fault inputs (staggering is done on the host)
+  std::vector<float> h_striniX, h_striniZ, h_peak, h_dyn, h_Dc, h_coh;
+  std::vector<double> h_a, h_b, h_psi, h_vw, h_aX, h_bX, h_psiX, h_vwX;
+  int *d_sta = nullptr;
+  // profiling
+  bool prof = false;
+  struct Ev { int id; hipEvent_t a, b; };
+  std::vector<Ev> evs;
+  std::vector<hipEvent_t> pool;
+  double kms[K_COUNT] = {0};
+  long kcount[K_COUNT] = {0};
+  // step sequence
+  std::vector<KernelId> seq;
+  // the batch this problem is advanced in (fd3d_batch_add), or null
+  fd3d_batch *batch = nullptr;
+};
+
+
+// The stream a batch runs its steps on (batch.hip).
+hipStream_t fd3d_batch_stream_of(const fd3d_batch *b);
+
+// Reads the device control block of h on stream s and turns its flags into
+// FD3D_E_NEWTON / FD3D_E_NONFINITE (capi.hip).
+int fd3d_handle_check_ctl(fd3d_handle *h, hipStream_t s);

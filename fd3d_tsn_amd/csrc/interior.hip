@@ -1,200 +1,24 @@
-// Tiled interior sweeps: the 4th-order velocity (dvel -> uxx1/vyy1/wzz1,
-// fd3d_deriv.f90:24-193) and stress (dstres, :513-622) updates over
-// i in [nabc+1, nxt-nabc], j in [nabc+1, nyt-2], k in [nabc+1, nzt-nfs].
-//
-// Layout of the work: a block owns a 64 x TZ tile of the x-z plane (one
-// wavefront along x per z row) and marches along y over NY rows.
-//  * y neighbours come from per-thread register windows (each value is
-//    loaded once per block);
-//  * x and z neighbours come from the x-z plane of the current row staged
-//    in LDS, double-buffered so one barrier per row suffices; halo cells
-//    (2 columns / 2 rows on each side) are loaded by the block, centre
-//    cells are written from the registers that already hold them;
-//  * blocks are numbered so that each XCD (block id modulo 8 on MI355X)
-//    receives a contiguous run of the logical tile order, keeping x and z
-//    neighbour tiles on the same L2.
-// Per-point arithmetic is the reference's, operand for operand (see
-// kernels.hip), so results are bit-identical to the untiled form.
-#include "fd3d_dev.h"
+// Single-problem launches of the tiled interior sweeps (tiles.h).  Blocks
+// are numbered so that each XCD (block id modulo 8 on MI355X) receives a
+// contiguous run of the logical tile order, keeping x and z neighbour
+// tiles on the same L2.
+#include "tiles.h"
 
 namespace fd3d {
 namespace {
 
-// Logical block id of hardware block pid when blocks are dealt round-robin
-// to 8 XCDs: XCD x gets the contiguous logical range [x*N/8, (x+1)*N/8).
-__device__ inline int xcd_remap(int pid, int N) {
-  const int q = N >> 3, r = N & 7;
-  const int xcd = pid & 7, loc = pid >> 3;
-  return xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+template <int TZ>
+__global__ void __launch_bounds__(64 * TZ) k_vel_tiled(Dims g, Coef c, Fields f, int gx, int gz, int gy, int NY) {
+  __shared__ float lds[Tile<TZ>::LDS_FLOATS];
+  const int lid = xcd_remap(blockIdx.x, gx * gz * gy);
+  vel_tiled_body<TZ>(g, c, f, lds, lid % gx, (lid / gx) % gz, lid / (gx * gz), threadIdx.x, threadIdx.y, NY);
 }
 
 template <int TZ>
-struct Tile {
-  static constexpr int W = 64 + 4, H = TZ + 4, N = W * H;
-  __device__ static int at(int x, int z) { return (z + 2) * W + (x + 2); }
-};
-
-// ---------------------------------------------------------------------------
-// Velocity.  Register windows along y: xy (j-2..j+1), yy (j-1..j+2),
-// yz (j-2..j+1).  LDS x-z plane at row j: xx (x halo), xy (x), xz (x, z),
-// yz (z), zz (z).
-// ---------------------------------------------------------------------------
-template <int TZ>
-__global__ void __launch_bounds__(64 * TZ) k_vel_tiled(Dims g, Coef c, Fields f, int gx, int gz, int gy,
-                                                       int NY) {
-  using T = Tile<TZ>;
-  __shared__ float lds[2][5][T::N];
+__global__ void __launch_bounds__(64 * TZ) k_str_tiled(Dims g, Coef c, Fields f, int gx, int gz, int gy, int NY) {
+  __shared__ float lds[2 * 4 * Tile<TZ>::N];
   const int lid = xcd_remap(blockIdx.x, gx * gz * gy);
-  const int bx = lid % gx, bz = (lid / gx) % gz, by = lid / (gx * gz);
-  const int tx = threadIdx.x, tz = threadIdx.y, t = tz * 64 + tx;
-  const int i0 = g.nabc + 1 + bx * 64, kk0 = g.nabc + 1 + bz * TZ;
-  const int i = i0 + tx, k = kk0 + tz;
-  const int j0 = g.nabc + 1 + by * NY;
-  const int je = min(j0 + NY - 1, g.nyt - 2);
-  const bool active = i <= g.nxt - g.nabc && k <= g.nzt - g.nfs;
-  const bool inside = i <= g.nxt && k <= g.nzt;
-  const int sy = g.sy, sz = g.sz;
-  const float c1 = c.c1, c2 = c.c2;
-  const float *xx = f.xx, *yy = f.yy, *zz = f.zz, *xy = f.xy, *yz = f.yz, *xz = f.xz;
-  float xym2 = 0, xym1 = 0, xy0 = 0, xyp1 = 0, yym1 = 0, yy0 = 0, yyp1 = 0, yyp2 = 0;
-  float yzm2 = 0, yzm1 = 0, yz0 = 0, yzp1 = 0;
-  if (inside) {
-    const int q = I3(g, i, j0, k);
-    xym2 = xy[q - 2 * sy]; xym1 = xy[q - sy]; xy0 = xy[q]; xyp1 = xy[q + sy];
-    yym1 = yy[q - sy]; yy0 = yy[q]; yyp1 = yy[q + sy]; yyp2 = yy[q + 2 * sy];
-    yzm2 = yz[q - 2 * sy]; yzm1 = yz[q - sy]; yz0 = yz[q]; yzp1 = yz[q + sy];
-  }
-  constexpr int NX = 4 * TZ, NZH = 4 * 68;
-  constexpr int NHTOT = NX + NX + (NX + NZH) + NZH + NZH;
-  int buf = 0;
-  for (int j = j0; j <= je; ++j) {
-    const int qj = I3(g, 0, j, 1);  // element (i,j,k) at qj + i + (k-1)*sz
-    float(*L)[T::N] = lds[buf];
-    const int a0 = T::at(tx, tz);
-    float xx0 = 0, xz0 = 0, zz0 = 0;
-    const int q = qj + i + (k - 1) * sz;
-    if (inside) { xx0 = xx[q]; xz0 = xz[q]; zz0 = zz[q]; }
-    L[0][a0] = xx0; L[1][a0] = xy0; L[2][a0] = xz0; L[3][a0] = yz0; L[4][a0] = zz0;
-    for (int h = t; h < NHTOT; h += 64 * TZ) {
-      int fld, e = h;
-      bool xh;
-      if (e < NX) { fld = 0; xh = true; }
-      else if ((e -= NX) < NX) { fld = 1; xh = true; }
-      else if ((e -= NX) < NX + NZH) { fld = 2; xh = e < NX; if (!xh) e -= NX; }
-      else if ((e -= NX + NZH) < NZH) { fld = 3; xh = false; }
-      else { e -= NZH; fld = 4; xh = false; }
-      int hx, hz;
-      if (xh) { hz = e >> 2; const int c4 = e & 3; hx = c4 < 2 ? c4 - 2 : 62 + c4; }
-      else { const int r = e / 68, cc = e % 68; hz = r < 2 ? r - 2 : TZ + r - 2; hx = cc - 2; }
-      const int gi = i0 + hx, gk = kk0 + hz;
-      const float *s = fld == 0 ? xx : fld == 1 ? xy : fld == 2 ? xz : fld == 3 ? yz : zz;
-      float val = 0.0f;
-      if (gi >= 1 && gi <= g.nxt && gk >= 1 && gk <= g.nzt) val = s[qj + gi + (gk - 1) * sz];
-      L[fld][T::at(hx, hz)] = val;
-    }
-    __syncthreads();
-    if (active) {
-      const float *Lxx = L[0], *Lxy = L[1], *Lxz = L[2], *Lyz = L[3], *Lzz = L[4];
-      const int W = T::W;
-      const float b = c.dth / f.d[q];
-      f.u[q] = f.u[q] + b * (c1 * (xx0 - Lxx[a0 - 1]) + c2 * (Lxx[a0 + 1] - Lxx[a0 - 2]) +
-                             c1 * (xy0 - xym1) + c2 * (xyp1 - xym2) +
-                             c1 * (xz0 - Lxz[a0 - W]) + c2 * (Lxz[a0 + W] - Lxz[a0 - 2 * W]));
-      f.v[q] = f.v[q] + b * (c1 * (Lxy[a0 + 1] - xy0) + c2 * (Lxy[a0 + 2] - Lxy[a0 - 1]) +
-                             c1 * (yyp1 - yy0) + c2 * (yyp2 - yym1) +
-                             c1 * (yz0 - Lyz[a0 - W]) + c2 * (Lyz[a0 + W] - Lyz[a0 - 2 * W]));
-      f.w[q] = f.w[q] + b * (c1 * (Lxz[a0 + 1] - xz0) + c2 * (Lxz[a0 + 2] - Lxz[a0 - 1]) +
-                             c1 * (yz0 - yzm1) + c2 * (yzp1 - yzm2) +
-                             c1 * (Lzz[a0 + W] - zz0) + c2 * (Lzz[a0 + 2 * W] - Lzz[a0 - W]));
-    }
-    if (j < je && inside) {
-      xym2 = xym1; xym1 = xy0; xy0 = xyp1; xyp1 = xy[q + 2 * sy];
-      yym1 = yy0; yy0 = yyp1; yyp1 = yyp2; yyp2 = yy[q + 3 * sy];
-      yzm2 = yzm1; yzm1 = yz0; yz0 = yzp1; yzp1 = yz[q + 2 * sy];
-    }
-    buf ^= 1;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Stress.  Register windows along y: v (j-2..j+1), u (j-1..j+2),
-// w (j-1..j+2), mu (j, j+1).  LDS x-z plane at row j: u, v, w, mu.
-// ---------------------------------------------------------------------------
-template <int TZ>
-__global__ void __launch_bounds__(64 * TZ) k_str_tiled(Dims g, Coef c, Fields f, int gx, int gz, int gy,
-                                                       int NY) {
-  using T = Tile<TZ>;
-  __shared__ float lds[2][4][T::N];
-  const int lid = xcd_remap(blockIdx.x, gx * gz * gy);
-  const int bx = lid % gx, bz = (lid / gx) % gz, by = lid / (gx * gz);
-  const int tx = threadIdx.x, tz = threadIdx.y, t = tz * 64 + tx;
-  const int i0 = g.nabc + 1 + bx * 64, kk0 = g.nabc + 1 + bz * TZ;
-  const int i = i0 + tx, k = kk0 + tz;
-  const int j0 = g.nabc + 1 + by * NY;
-  const int je = min(j0 + NY - 1, g.nyt - 2);
-  const bool active = i <= g.nxt - g.nabc && k <= g.nzt - g.nfs;
-  const bool inside = i <= g.nxt && k <= g.nzt;
-  const int sy = g.sy, sz = g.sz;
-  const float c1 = c.c1, c2 = c.c2, dth = c.dth;
-  const float *u = f.u, *v = f.v, *w = f.w, *mu = f.mu;
-  float vm2 = 0, vm1 = 0, v0 = 0, vp1 = 0, um1 = 0, u0 = 0, up1 = 0, up2 = 0;
-  float wm1 = 0, w0 = 0, wp1 = 0, wp2 = 0, m0 = 0, mp1 = 0;
-  if (inside) {
-    const int q = I3(g, i, j0, k);
-    vm2 = v[q - 2 * sy]; vm1 = v[q - sy]; v0 = v[q]; vp1 = v[q + sy];
-    um1 = u[q - sy]; u0 = u[q]; up1 = u[q + sy]; up2 = u[q + 2 * sy];
-    wm1 = w[q - sy]; w0 = w[q]; wp1 = w[q + sy]; wp2 = w[q + 2 * sy];
-    m0 = mu[q]; mp1 = mu[q + sy];
-  }
-  constexpr int NH = 4 * TZ + 4 * 68;
-  int buf = 0;
-  for (int j = j0; j <= je; ++j) {
-    const int qj = I3(g, 0, j, 1);
-    float(*L)[T::N] = lds[buf];
-    const int a0 = T::at(tx, tz);
-    const int q = qj + i + (k - 1) * sz;
-    L[0][a0] = u0; L[1][a0] = v0; L[2][a0] = w0; L[3][a0] = m0;
-    for (int h = t; h < 4 * NH; h += 64 * TZ) {
-      const int fld = h / NH, e = h - fld * NH;
-      int hx, hz;
-      if (e < 4 * TZ) { hz = e >> 2; const int c4 = e & 3; hx = c4 < 2 ? c4 - 2 : 62 + c4; }
-      else { const int r = (e - 4 * TZ) / 68, cc = (e - 4 * TZ) % 68; hz = r < 2 ? r - 2 : TZ + r - 2; hx = cc - 2; }
-      const int gi = i0 + hx, gk = kk0 + hz;
-      const float *s = fld == 0 ? u : fld == 1 ? v : fld == 2 ? w : mu;
-      float val = 0.0f;
-      if (gi >= 1 && gi <= g.nxt && gk >= 1 && gk <= g.nzt) val = s[qj + gi + (gk - 1) * sz];
-      L[fld][T::at(hx, hz)] = val;
-    }
-    __syncthreads();
-    if (active) {
-      const float *Lu = L[0], *Lv = L[1], *Lw = L[2], *Lm = L[3];
-      const int W = T::W;
-      const float xl = f.lam[q], xm = m0;
-      const float a = xl + 2.0f * xm, b = xl;
-      const float dux1 = c1 * (Lu[a0 + 1] - u0), dux2 = c2 * (Lu[a0 + 2] - Lu[a0 - 1]);
-      const float dvy1 = c1 * (v0 - vm1), dvy2 = c2 * (vp1 - vm2);
-      const float dwz1 = c1 * (w0 - Lw[a0 - W]), dwz2 = c2 * (Lw[a0 + W] - Lw[a0 - 2 * W]);
-      f.xx[q] = (f.xx[q] + (dth * a) * (dux1 + dux2)) + (dth * b) * (((dvy1 + dvy2) + dwz1) + dwz2);
-      f.yy[q] = (f.yy[q] + (dth * a) * (dvy1 + dvy2)) + (dth * b) * (((dux1 + dux2) + dwz1) + dwz2);
-      f.zz[q] = (f.zz[q] + (dth * a) * (dwz1 + dwz2)) + (dth * b) * (((dux1 + dux2) + dvy1) + dvy2);
-      float xmu = 0.5f * (xm + mp1);
-      f.xy[q] = f.xy[q] + (dth * xmu) * (((c1 * (up1 - u0) + c2 * (up2 - um1)) + c1 * (v0 - Lv[a0 - 1])) +
-                                         c2 * (Lv[a0 + 1] - Lv[a0 - 2]));
-      xmu = 0.5f * (xm + Lm[a0 + W]);
-      f.xz[q] = f.xz[q] + (dth * xmu) * (((c1 * (Lu[a0 + W] - u0) + c2 * (Lu[a0 + 2 * W] - Lu[a0 - W])) +
-                                          c1 * (w0 - Lw[a0 - 1])) + c2 * (Lw[a0 + 1] - Lw[a0 - 2]));
-      xmu = 0.5f * (xm + mu[q + 1 + sy + sz]);
-      f.yz[q] = f.yz[q] + (dth * xmu) * (((c1 * (Lv[a0 + W] - v0) + c2 * (Lv[a0 + 2 * W] - Lv[a0 - W])) +
-                                          c1 * (wp1 - w0)) + c2 * (wp2 - wm1));
-    }
-    if (j < je && inside) {
-      vm2 = vm1; vm1 = v0; v0 = vp1; vp1 = v[q + 2 * sy];
-      um1 = u0; u0 = up1; up1 = up2; up2 = u[q + 3 * sy];
-      wm1 = w0; w0 = wp1; wp1 = wp2; wp2 = w[q + 3 * sy];
-      m0 = mp1; mp1 = mu[q + 2 * sy];
-    }
-    buf ^= 1;
-  }
+  str_tiled_body<TZ>(g, c, f, lds, lid % gx, (lid / gx) % gz, lid / (gx * gz), threadIdx.x, threadIdx.y, NY);
 }
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }

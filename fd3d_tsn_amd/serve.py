@@ -1,0 +1,217 @@
+"""Serving loop for many forward rupture models (continuous batching).
+
+A dynamic inversion evaluates many forward models of FD3D_TSN, each one a
+run of the reference's whole program (dynamicsolver.f90:66 -> fd3d()).
+Here requests of different grid sizes, friction laws and fault fields
+arrive as a stream and share one GPU:
+
+* a request is admitted when a batch slot is free and its device
+  footprint (fd3d_footprint) fits in free HBM (fd3d_device_memory) minus a
+  reserve; its wavefield then stays resident until it finishes;
+* admitted problems are advanced together by a :class:`capi.Batch` (one
+  launch per stage covering every problem, the step replayed from a HIP
+  graph), in chunks of steps no longer than the shortest remaining run;
+* while a chunk runs on the GPU, the host prepares the next requests
+  (inputs, device allocation, upload, prologue) on their own streams; they
+  join at the next chunk boundary, when finished problems leave;
+* each result (the final 2D arrays of fd3d(), fd3d_theo.f90:604-658) is
+  bit-identical to the same request solved alone.
+
+:func:`run_plain` serves the same stream back to back, one request at a
+time through fd3d_step, as the baseline.
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from . import capi
+from . import cases as C
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds after the start of the stream
+    case: C.Case
+
+    @property
+    def steps(self) -> int:
+        return self.case.ntfd
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float
+    finished: float
+    steps: int
+    cells: int
+    final: Dict[str, np.ndarray] = field(repr=False, default_factory=dict)
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def c5_requests(n: int = 2000, rate: float = 20.0, seed: int = 5) -> List[Request]:
+    """Poisson stream of rupture requests (mean `rate` per second): 70% slip
+    weakening, 30% rate-and-state; interior grids from 64^3 to 384x192x192,
+    dh 100-200 m, 800-3000 steps, dt at CFL 0.24; per-request seeded
+    prestress and friction fields.  Chosen where the workload leaves it
+    open: 10-node PML; slip weakening in a full space with the batched
+    workload's fault recipe; rate-and-state with a free surface (the TPV104
+    layout) and the nucleation at a seeded point of the fault box."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    out = []
+    for q in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        fvw = bool(rng.random() < 0.3)
+        nx = int(rng.integers(64, 385))
+        ny, nz = (int(v) for v in rng.integers(64, 193, size=2))
+        dh = float(round(rng.uniform(100.0, 200.0)))
+        steps = int(rng.integers(800, 3001))
+        dt = float(np.float32(0.24 * dh / 6000.0))
+        s = seed * 100000 + q
+        flen, fwid = 0.8 * nx * dh, 0.8 * nz * dh
+        if fvw:
+            case = C.Case(f"req{q:04d}_fvw_{nx}x{ny}x{nz}", nx, ny, nz, dh=dh, dt=dt, ntfd=steps, nabc=10,
+                          fspace=False, friction="fvw", seed=s, fault_len=flen, fault_wid=fwid,
+                          trans=min(3e3, 0.1 * flen), nuc_radius=min(3e3, 0.25 * fwid))
+        else:
+            mus = float(rng.uniform(0.6, 0.75))
+            dc = float(rng.uniform(0.2, 0.8))
+            fx, fz = rng.uniform(-0.25, 0.25, size=2)
+            sn, mud = 120e6, 0.525
+            xc, zc = 0.5 * (nx - 1) * dh, 0.5 * (nz - 1) * dh
+            case = C.Case(f"req{q:04d}_sw_{nx}x{ny}x{nz}", nx, ny, nz, dh=dh, dt=dt, ntfd=steps, nabc=10,
+                          fspace=True, seed=s, fault_len=flen, fault_wid=fwid, mus=mus, mud=mud, dc=dc, sn=sn,
+                          t0=sn * (mud + 0.6 * (mus - mud)), t0_std=2e6, t0_corr=2e3,
+                          nuc_size=min(3e3, 0.3 * fwid), nuc_stress=sn * mus + 1.5e6,
+                          nuc_xc=xc + float(fx) * flen, nuc_zc=zc + float(fz) * fwid)
+        out.append(Request(q, t, case))
+    return out
+
+
+def percentile(x, p):
+    return float(np.percentile(np.asarray(x, dtype=np.float64), p)) if len(x) else float("nan")
+
+
+def summary(results: List[Result], makespan: float) -> Dict[str, float]:
+    lat = [r.latency for r in results]
+    upd = sum(r.cells * r.steps for r in results)
+    return {"requests": len(results), "makespan_s": makespan,
+            "ruptures_per_s": len(results) / makespan if makespan > 0 else float("nan"),
+            "Gcell_updates_per_s": upd / makespan / 1e9 if makespan > 0 else float("nan"),
+            "latency_p50_s": percentile(lat, 50), "latency_p99_s": percentile(lat, 99),
+            "latency_mean_s": float(np.mean(lat)) if lat else float("nan")}
+
+
+class Server:
+    """Continuous-batching server on one GPU."""
+
+    def __init__(self, max_slots: int = 32, chunk: int = 16, tz: int = 16, mode: int = capi.STAGED,
+                 reserve_bytes: int = 4 << 30, keep_final: bool = True,
+                 log: Optional[Callable[[str], None]] = None):
+        self.max_slots = max_slots
+        self.chunk = chunk
+        self.reserve = reserve_bytes
+        self.keep_final = keep_final
+        self.batch = capi.Batch(max_slots, tz=tz, mode=mode)
+        self.log = log or (lambda m: None)
+        self.gpu_busy_s = 0.0
+
+    def close(self):
+        self.batch.close()
+
+    def _fits(self, req: Request, pending_bytes: int) -> bool:
+        free, _total = capi.device_memory()
+        return capi.footprint(req.case) + pending_bytes + self.reserve <= free
+
+    def run(self, requests: List[Request], clock=time.perf_counter) -> List[Result]:
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        queue: collections.deque = collections.deque()
+        active: List[dict] = []
+        ready: List[dict] = []          # prepared during the last chunk, join at the next boundary
+        results: List[Result] = []
+        t0 = clock()
+        while pending or queue or active or ready:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.popleft())
+            for a in ready:
+                self.batch.add(a["solver"])
+                active.append(a)
+            ready = []
+            n = min([self.chunk] + [a["left"] for a in active]) if active else 0
+            if n:
+                self.batch.step(n)          # asynchronous: the host prepares admissions meanwhile
+            # admission: a free slot and the footprint in free HBM
+            while queue and len(active) + len(ready) < self.max_slots and self._fits(queue[0], 0):
+                r = queue.popleft()
+                ready.append({"req": r, "solver": capi.Solver(r.case), "left": r.steps,
+                              "admitted": clock() - t0})
+            if queue and not active and not ready:
+                raise RuntimeError(f"request {queue[0].rid} does not fit in device memory")
+            if n:
+                t1 = clock()
+                self.batch.sync()
+                self.gpu_busy_s += clock() - t1
+                done = []
+                for a in active:
+                    a["left"] -= n
+                    if a["left"] == 0:
+                        done.append(a)
+                for a in done:
+                    s = a["solver"]
+                    self.batch.remove(s)
+                    fin = s.final() if self.keep_final else {}
+                    s.close()
+                    r = a["req"]
+                    results.append(Result(r.rid, r.arrival, a["admitted"], clock() - t0, r.steps, r.case.cells,
+                                          fin))
+                    active.remove(a)
+                    self.log(f"[serve] request {r.rid} done: latency {results[-1].latency:.3f} s, "
+                             f"{len(active)} active, {len(queue)} queued")
+            elif not ready and pending:
+                time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
+        self.makespan = clock() - t0
+        return results
+
+
+def run_plain(requests: List[Request], keep_final: bool = True, clock=time.perf_counter,
+              log: Optional[Callable[[str], None]] = None) -> List[Result]:
+    """Back-to-back plain solves in arrival order (one request at a time).
+    Service times are measured; the queueing timeline of a FIFO single
+    server then follows from the arrival times, so the run does not wait
+    for arrivals in real time."""
+    results = []
+    free_at = 0.0
+    for r in sorted(requests, key=lambda q: q.arrival):
+        t1 = clock()
+        with capi.Solver(r.case) as s:
+            s.step(r.steps, outputs=False)
+            s.sync()
+            fin = s.final() if keep_final else {}
+        service = clock() - t1
+        start = max(free_at, r.arrival)
+        free_at = start + service
+        results.append(Result(r.rid, r.arrival, start, free_at, r.steps, r.case.cells, fin))
+        if log:
+            log(f"[plain] request {r.rid}: {service:.3f} s")
+    return results
+
+
+def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    m = np.isfinite(a) & np.isfinite(b)
+    den = np.sqrt(np.sum(b[m] ** 2))
+    num = np.sqrt(np.sum((a[m] - b[m]) ** 2))
+    return float(num / den) if den > 0 else float(num)

@@ -95,6 +95,10 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out);
 /* Device bytes fd3d_create would allocate for cfg (admission control). */
 size_t fd3d_footprint(const fd3d_config *cfg);
 
+/* Free and total device memory of the current GPU in bytes (admission of
+ * new problems against free HBM); the reference has no counterpart. */
+int fd3d_device_memory(size_t *free_bytes, size_t *total_bytes);
+
 /* medium_com arrays lam1, mu1, d1 (fd3d_init.f90:262-264), nxt*nyt*nzt each.
  * Returns FD3D_E_CFL when max(Vp)*dt/dh > 0.25 (fd3d_init.f90:274-278). */
 int fd3d_upload_medium(fd3d_handle *h, const float *lam1, const float *mu1, const float *d1);
@@ -133,6 +137,14 @@ int fd3d_sync(fd3d_handle *h);
 /* Steps taken since fd3d_init_state. */
 int fd3d_steps_done(const fd3d_handle *h);
 
+/* Copy the output slabs (out, FD3D_NOUT*nxtT*nztT floats per step) and
+ * station samples (seis, nstations*3 floats per step) of the last nsteps
+ * steps from the device ring, oldest first; nsteps <= ring_steps.  Either
+ * pointer may be NULL.  The same data fd3d_step hands back per call, for
+ * problems advanced by fd3d_batch_step (the reference writes them every
+ * step, fd3d_theo.f90:546-557). */
+int fd3d_read_outputs(fd3d_handle *h, int nsteps, float *out, float *seis);
+
 /* The arrays fd3d() leaves in source_com after the loop (fd3d_theo.f90:604-658),
  * in fd3d_final order, each nxt*nzt: ruptime, rise (minus ruptime), sliptime,
  * slipX, slipZ, schangeX-T0X, schangeZ-T0Z.  out7 holds 7*nxt*nzt floats. */
@@ -162,7 +174,59 @@ int fd3d_kernel_times(fd3d_handle *h, int n, const char **names, double *ms, lon
 double fd3d_step_bytes(const fd3d_handle *h, long *bulk_cells, long *pml_cells);
 
 const char *fd3d_last_error(const fd3d_handle *h);
+
+/* Release the handle and its device memory (the deallocations at
+ * fd3d_theo.f90:609-625); removes it from its batch first. */
 void fd3d_destroy(fd3d_handle *h);
+
+/* ------------------------------------------------------------------------
+ * Batched problems.  The reference runs one rupture per process
+ * (dynamicsolver.f90:66 calls fd3d() once); a dynamic inversion runs many
+ * such forward models back to back.  A batch advances many resident
+ * problems of different sizes, friction laws and free-surface settings
+ * together: each time step is one launch in which workgroups take
+ * (problem, stage, block) items from a device queue, a stage of a problem
+ * starting when the stage before it in that problem has finished; the step
+ * is captured once in a HIP graph and replayed.  Results are bit-identical
+ * to fd3d_step on each problem alone.
+ * ------------------------------------------------------------------------ */
+typedef struct fd3d_batch fd3d_batch;
+
+enum fd3d_batch_mode {
+  FD3D_BATCH_PERSISTENT = 0,  /* one persistent launch per step, dependency counters per problem;
+                                 currently refused with FD3D_E_STATE (stalled on MI355X) */
+  FD3D_BATCH_STAGED = 1       /* one launch per stage covering every problem */
+};
+
+/* tz: x-z tile height of the interior sweeps of every member (4, 8 or 16);
+ * each member keeps its own rows-per-tile (ny_vel, ny_str of fd3d_set_tuning).
+ * mode FD3D_BATCH_PERSISTENT is refused with FD3D_E_STATE for now. */
+int fd3d_batch_create(int max_problems, int tz, int mode, fd3d_batch **out);
+
+/* Add an initialised problem (fd3d_init_state done).  While it is a member,
+ * fd3d_step, fd3d_set_tuning and fd3d_init_state on it fail with
+ * FD3D_E_STATE; downloads and fd3d_read_outputs wait for the batch. */
+int fd3d_batch_add(fd3d_batch *b, fd3d_handle *h);
+int fd3d_batch_remove(fd3d_batch *b, fd3d_handle *h);
+int fd3d_batch_size(const fd3d_batch *b);
+
+/* Advance every member nsteps time steps (asynchronous; fd3d_batch_sync). */
+int fd3d_batch_step(fd3d_batch *b, int nsteps);
+
+/* Wait for queued steps; reports a failed dependency wait (FD3D_E_HIP) and
+ * the members' FD3D_E_NEWTON / FD3D_E_NONFINITE flags. */
+int fd3d_batch_sync(fd3d_batch *b);
+
+/* out4: resident workgroups of the persistent launch, work items per step,
+ * (problem, stage) segments per step, kernel launches per step. */
+int fd3d_batch_info(const fd3d_batch *b, int *out4);
+
+/* Milliseconds of GPU time of the steps since the last call (HIP events
+ * around each replayed step graph); *nsteps receives the step count. */
+double fd3d_batch_elapsed(fd3d_batch *b, int *nsteps);
+
+const char *fd3d_batch_last_error(const fd3d_batch *b);
+void fd3d_batch_destroy(fd3d_batch *b);
 
 #ifdef __cplusplus
 }

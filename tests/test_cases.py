@@ -70,3 +70,31 @@ def test_fvw_setup_is_steady_state_consistent():
     # tau = Sn * a * asinh(V/(2 V0) exp(psi/a)) with V = 2*vini
     tau = sc["Sn"] * a * np.arcsinh((2 * c.vini) / (2 * c.v0) * np.exp(psi / a))
     assert np.allclose(tau, c.tini, rtol=1e-9)
+
+
+def test_batched_workload_shape():
+    cs = C.c4_batched(64, seed=4)
+    assert len(cs) == 64 and len({c.name for c in cs}) == 64
+    for c in cs:
+        assert 96 <= c.nxtT <= 320 and 96 <= c.nytT <= 320 and 96 <= c.nztT <= 320
+        assert c.dh == 100.0 and c.ntfd == 1500 and c.nabc == 10 and c.fspace
+        assert abs(c.cfl() - 0.24) < 1e-6
+        assert 0.2 <= c.dc <= 0.8 and 0.6 <= c.mus <= 0.75
+        f = C.fault_sw(c.replace(nxtT=c.nxtT, nytT=8))
+        # nucleation patch above static strength, prestress below it on average
+        assert f["striniX"].max() >= c.sn * c.mus
+        assert np.mean(f["striniX"]) < c.sn * c.mus
+
+
+def test_request_stream_shape():
+    from fd3d_tsn_amd import serve
+    reqs = serve.c5_requests(200, seed=5)
+    fr = np.mean([r.case.friction == "fvw" for r in reqs])
+    assert 0.2 < fr < 0.4
+    gaps = np.diff([r.arrival for r in reqs])
+    assert 0.03 < gaps.mean() < 0.07
+    for r in reqs:
+        c = r.case
+        assert 64 <= c.nxtT <= 384 and 64 <= c.nytT <= 192 and 64 <= c.nztT <= 192
+        assert 800 <= c.ntfd <= 3000 and 100 <= c.dh <= 200
+        assert abs(c.cfl() - 0.24) < 1e-3
