@@ -40,10 +40,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="large_sw", choices=["large_sw", "plain_sw", "fvw", "batched"])
+    ap.add_argument("--workload", default="large_sw", choices=["large_sw", "plain_sw", "fvw", "batched", "served"])
     ap.add_argument("--batch-mode", default="staged", choices=["persistent", "staged"])
     ap.add_argument("--batch-tz", type=int, default=16)
     ap.add_argument("--batch-n", type=int, default=64, help="domains of the batched workload")
+    ap.add_argument("--serve-n", type=int, default=48, help="requests of the served stream sample")
+    ap.add_argument("--serve-slots", type=int, default=32)
+    ap.add_argument("--serve-chunk", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="time steps of the reference CPU build on the same grid")
@@ -112,31 +115,43 @@ def bench_batched(a):
     1500 steps each): time per step of the batch against the sum of the
     per-step times of each domain stepped alone (back-to-back plain
     solves); forward ruptures/s follow from the 1500-step length."""
+    from fd3d_tsn_amd import autotune
     from fd3d_tsn_amd import capi
     from fd3d_tsn_amd import cases as C
     cases = C.c4_batched(a.batch_n)
+    tz, plan = autotune.batch_plan(cases)
+    tuned = sum(p is not None for p in plan)
     nsteps_run = cases[0].ntfd
     t0 = time.perf_counter()
     solvers = []
     setup_s = []
+    phases = {}
     for c in cases:
         t1 = time.perf_counter()
         solvers.append(capi.Solver(c))
         setup_s.append(time.perf_counter() - t1)
+        for k, v in solvers[-1].setup_times.items():
+            phases[k] = phases.get(k, 0.0) + v / len(cases)
     print(f"[bench] batched: {len(cases)} domains, {sum(c.cells for c in cases)/1e6:.0f}M cells, "
           f"setup {time.perf_counter()-t0:.1f} s", file=sys.stderr, flush=True)
     plain = []
     for s in solvers:
+        autotune.apply(s)            # each problem alone gets its own best shape
         s.step(a.warmup, outputs=False)
         s.sync()
         t1 = time.perf_counter()
         s.step(a.steps, outputs=False)
         s.sync()
         plain.append((time.perf_counter() - t1) / a.steps)
-    for s in solvers:
+    step_b = [s.step_bytes()[0] for s in solvers]
+    for s, p in zip(solvers, plan):
         s.init()
+        if p is not None:
+            s.set_tuning(*p)
     mode = capi.PERSISTENT if a.batch_mode == "persistent" else capi.STAGED
-    b = capi.Batch(len(solvers), tz=a.batch_tz, mode=mode)
+    if not tuned:
+        tz = a.batch_tz
+    b = capi.Batch(len(solvers), tz=tz, mode=mode)
     for s in solvers:
         b.add(s)
     b.step(a.warmup)
@@ -153,6 +168,7 @@ def bench_batched(a):
         s.close()
     cells = sum(c.cells for c in cases)
     t_plain = sum(plain)
+    step_bytes = sum(step_b)
     value = len(cases) / (nsteps_run * wall)
     out = {"metric": "forward ruptures/s (batched uneven domains)", "value": value,
            "unit": "ruptures/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -160,14 +176,59 @@ def bench_batched(a):
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"batched_{len(cases)}_domains_96-320", "domains": len(cases),
                       "cells": cells, "steps_per_rupture": nsteps_run, "mode": a.batch_mode,
-                      "tz": a.batch_tz},
+                      "tz": tz, "autotuned_members": tuned},
            "batch": {"gpu_ms_per_step": gpu_ms / max(n, 1), "Gcell_updates_per_s": cells / wall / 1e9,
-                     "info": info},
+                     "hbm_frac": step_bytes / wall / 1e9 / HBM_PEAK_GBS, "info": info},
            "plain_back_to_back": {"ms_per_step_sum": t_plain * 1e3,
                                   "ruptures_per_s": len(cases) / (nsteps_run * t_plain),
                                   "Gcell_updates_per_s": cells / t_plain / 1e9,
-                                  "setup_s_per_rupture": sum(setup_s) / len(setup_s)},
+                                  "hbm_frac": step_bytes / t_plain / 1e9 / HBM_PEAK_GBS,
+                                  "setup_s_per_rupture": sum(setup_s) / len(setup_s),
+                                  "setup_phases_s": phases},
            "speedup_vs_plain": t_plain / wall}
+    print(json.dumps(out))
+
+
+def bench_served(a):
+    """Serving loop on the first --serve-n requests of the Poisson stream
+    (mean 20 requests/s, 70% slip weakening / 30% rate-and-state, grids
+    64^3..384x192x192, 800-3000 steps): continuous batching against the same
+    requests solved back to back; per-request relative L2 of the final
+    arrays between the two."""
+    from fd3d_tsn_amd import capi
+    from fd3d_tsn_amd import serve
+    reqs = serve.c5_requests(2000)[:a.serve_n]
+    upd = sum(r.case.cells * r.steps for r in reqs)
+    print(f"[bench] served: {len(reqs)} requests, {upd/1e9:.0f} G cell-updates", file=sys.stderr, flush=True)
+    # warm the library and the device once
+    with capi.Solver(reqs[0].case.replace(nxtT=32, nytT=16, nztT=16, ntfd=4)) as s:
+        s.step(4, outputs=False)
+        s.sync()
+    t1 = time.perf_counter()
+    plain = serve.run_plain(reqs)
+    print(f"[bench] plain back-to-back {time.perf_counter()-t1:.1f} s", file=sys.stderr, flush=True)
+    srv = serve.Server(max_slots=a.serve_slots, chunk=a.serve_chunk, tz=a.batch_tz)
+    t1 = time.perf_counter()
+    got = srv.run(reqs)
+    print(f"[bench] served {time.perf_counter()-t1:.1f} s", file=sys.stderr, flush=True)
+    srv.close()
+    ps = serve.summary(plain, max(r.finished for r in plain))
+    ss = serve.summary(got, srv.makespan)
+    pmap = {r.rid: r for r in plain}
+    l2 = 0.0
+    for r in got:
+        for k, v in pmap[r.rid].final.items():
+            l2 = max(l2, serve.rel_l2(r.final[k], v))
+    out = {"metric": "served ruptures/s (Poisson stream, continuous batching)", "value": ss["ruptures_per_s"],
+           "unit": "ruptures/s", "n_gpus": 1, "steps": None, "warmup": 0,
+           "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 (rate-and-state Newton and state in f64)", "data": "synthetic",
+           "config": {"workload": f"served_stream_first_{len(reqs)}_of_2000", "requests": len(reqs),
+                      "rate_per_s": 20.0, "slots": a.serve_slots, "chunk_steps": a.serve_chunk,
+                      "cell_updates": upd},
+           "served": ss, "plain_back_to_back": ps,
+           "speedup_vs_plain": ss["ruptures_per_s"] / ps["ruptures_per_s"],
+           "max_rel_l2_vs_plain": l2}
     print(json.dumps(out))
 
 
@@ -179,6 +240,8 @@ def main():
         sys.exit(2)
     if a.workload == "batched":
         return bench_batched(a)
+    if a.workload == "served":
+        return bench_served(a)
     import numpy as np
     from fd3d_tsn_amd import capi
     from fd3d_tsn_amd import cases as C
@@ -197,6 +260,8 @@ def main():
     t0 = time.perf_counter()
     fault, scal = (C.fault_fvw(case) if case.friction == "fvw" else (None, None))
     s = capi.Solver(case, fault=fault, fvw_scalars=scal)
+    from fd3d_tsn_amd import autotune
+    tuned = autotune.apply(s)
     print(f"[bench] setup {time.perf_counter()-t0:.1f} s", file=sys.stderr, flush=True)
     s.step(a.warmup, outputs=False)
     s.sync()
@@ -208,6 +273,7 @@ def main():
     kt = s.kernel_times()
     s.set_profiling(False)
     step_bytes, nbulk, npml = s.step_bytes()
+    tuning = s.get_tuning()
     s.close()
 
     ms_per_step = elapsed * 1e3 / a.steps
@@ -237,7 +303,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": case.name, "grid": [case.nxt, case.nyt, case.nzt], "dh": case.dh,
                    "dt": case.dt, "nabc": case.nabc, "friction": case.friction,
-                   "fspace": case.fspace, "cells": case.cells},
+                   "fspace": case.fspace, "cells": case.cells, "tuning": list(tuning),
+                   "autotuned": tuned},
         "hbm_step": {"alg_bytes_per_step": step_bytes, "bulk_cells": nbulk, "pml_cells": npml,
                      "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                      "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},

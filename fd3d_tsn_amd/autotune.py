@@ -1,0 +1,186 @@
+"""Autotuner of the launch shapes of the 3D sweeps, per problem size class.
+
+The interior sweeps (tiles.h) take a tile height tz (4, 8, 16 lanes along
+z, 64 along x) and a number ny of rows each block marches along y; the
+PML sweeps run one thread per cell or march z in register windows.  The
+best shape depends on the grid: narrow grids waste lanes on tall tiles,
+short y extents leave too few blocks when ny is large.  Results never
+depend on the shape (the kernels are bit-identical across shapes), so a
+problem can be tuned live, in a few steps, by timing every candidate
+with HIP events on its own stream.
+
+Measured tables are cached per device and size class in a JSON file
+(``tuning_cache.json`` next to this module, or $FD3D_TUNING_CACHE).  A
+size class rounds each interior extent to the nearest of a fixed ladder
+of sizes and keeps the PML width and the free-surface setting.
+
+* :func:`tune` measures one problem and stores its table;
+* :func:`choose` returns the best (tz, ny) of a problem's class, or the
+  best ny for a given tz (the batch engine uses one tz for all members);
+* :func:`apply` sets a solver's shape from the cache (or leaves the
+  library's default when the class was never measured);
+* :func:`batch_plan` picks the common tz of a batch that minimises the
+  summed sweep time of its members, and each member's ny for that tz.
+"""
+from __future__ import annotations
+
+import json
+import os
+from typing import Dict, Iterable, List, Optional, Tuple
+
+from . import capi
+from . import cases as C
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.environ.get("FD3D_TUNING_CACHE", os.path.join(HERE, "tuning_cache.json"))
+LADDER = (32, 48, 64, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024, 1536, 2048)
+TZS = (4, 8, 16)
+NYS = (8, 16, 32, 64, 128)
+
+
+def size_class(case: C.Case) -> str:
+    def near(n):
+        return min(LADDER, key=lambda v: abs(v - n) / v)
+    return f"{near(case.nxtT)}x{near(case.nytT)}x{near(case.nztT)}/pml{case.nabc}/{'fs' if case.fspace else 'free'}"
+
+
+def _load() -> Dict:
+    if os.path.exists(CACHE):
+        with open(CACHE) as f:
+            return json.load(f)
+    return {"entries": {}}
+
+
+def _save(db: Dict):
+    tmp = CACHE + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(db, f, indent=1, sort_keys=True)
+    os.replace(tmp, CACHE)
+
+
+def candidates(case: C.Case) -> List[Tuple[int, int]]:
+    nyi = case.nyt - 2 - case.nabc
+    out = []
+    for tz in TZS:
+        for ny in NYS:
+            if ny <= max(8, 2 * nyi):
+                out.append((tz, ny))
+    return out
+
+
+def tune(solver: "capi.Solver", steps: int = 3, warmup: int = 1, save: bool = True) -> Dict:
+    """Time every candidate shape on a live problem (its state advances by
+    the steps taken; results do not depend on the shape).  Returns and
+    caches {"vel": {"tz x ny": ms}, "str": {...}, "pml": {"flat": ms, "zmarch": ms}}."""
+    case = solver.case
+    before = solver.get_tuning()
+    table = {"vel": {}, "str": {}, "pml": {}}
+    for tz, ny in candidates(case):
+        for pml_mode in ((1, 3) if (tz, ny) == (16, 32) else (1,)):
+            solver.set_tuning(pml_mode, tz, ny, tz, ny)
+            solver.step(warmup, outputs=False)
+            solver.set_profiling(True)
+            solver.step(steps, outputs=False)
+            kt = solver.kernel_times()
+            solver.set_profiling(False)
+            key = f"{tz}x{ny}"
+            if pml_mode == 1:
+                table["vel"][key] = kt["vel_interior"][0] / kt["vel_interior"][1]
+                table["str"][key] = kt["str_interior"][0] / kt["str_interior"][1]
+            if "pml_vel" in kt:
+                pm = (kt["pml_vel"][0] / kt["pml_vel"][1]) + (kt["pml_str"][0] / kt["pml_str"][1])
+                table["pml"]["zmarch" if pml_mode & 2 else "flat"] = pm
+    solver.set_tuning(*before)
+    if save:
+        db = _load()
+        db.setdefault("entries", {})[size_class(case)] = table
+        _save(db)
+    return table
+
+
+def _best(tab: Dict[str, float], tz: Optional[int] = None) -> Tuple[int, int, float]:
+    best = None
+    for k, ms in tab.items():
+        t, n = (int(v) for v in k.split("x"))
+        if tz is not None and t != tz:
+            continue
+        if best is None or ms < best[2]:
+            best = (t, n, ms)
+    return best
+
+
+def lookup(case: C.Case) -> Optional[Dict]:
+    return _load().get("entries", {}).get(size_class(case))
+
+
+def choose(case: C.Case, tz: Optional[int] = None) -> Optional[Tuple[int, int, int, int, int]]:
+    """(mode, tz_vel, ny_vel, tz_str, ny_str) from the cache, or None."""
+    e = lookup(case)
+    if not e or not e["vel"]:
+        return None
+    tv, nv, _ = _best(e["vel"], tz) or (None, None, None)
+    ts, ns, _ = _best(e["str"], tz) or (None, None, None)
+    if tv is None or ts is None:
+        return None
+    mode = 1
+    if e.get("pml") and e["pml"].get("zmarch", 1e30) < e["pml"].get("flat", 1e30):
+        mode = 3
+    return (mode, tv, nv, ts, ns)
+
+
+def apply(solver: "capi.Solver", tz: Optional[int] = None) -> bool:
+    t = choose(solver.case, tz)
+    if t is None:
+        return False
+    solver.set_tuning(*t)
+    return True
+
+
+def batch_plan(cases: Iterable[C.Case]) -> Tuple[int, List[Optional[Tuple[int, int, int, int, int]]]]:
+    """Common tile height for a batch and each member's shape at that height."""
+    cases = list(cases)
+    totals = {}
+    for tz in TZS:
+        tot = 0.0
+        for c in cases:
+            e = lookup(c)
+            if not e:
+                continue
+            bv, bs = _best(e["vel"], tz), _best(e["str"], tz)
+            if bv and bs:
+                tot += bv[2] + bs[2]
+        totals[tz] = tot
+    tz = min(TZS, key=lambda t: totals[t]) if any(totals.values()) else 16
+    return tz, [choose(c, tz) for c in cases]
+
+
+def main():
+    """python -m fd3d_tsn_amd.autotune [batched|served N|large] : tune the
+    size classes of a workload on the GPU and update the cache."""
+    import sys
+    import time
+    which = sys.argv[1] if len(sys.argv) > 1 else "batched"
+    n = int(sys.argv[2]) if len(sys.argv) > 2 else 48
+    if which == "batched":
+        cs = C.c4_batched()
+    elif which == "served":
+        from . import serve
+        cs = [r.case for r in serve.c5_requests(2000)[:n]]
+    else:
+        cs = [C.c2_large_sw()]
+    seen = set()
+    for c in cs:
+        k = size_class(c)
+        if k in seen:
+            continue
+        seen.add(k)
+        t0 = time.perf_counter()
+        with capi.Solver(c.replace(ntfd=8, stations=[])) as s:
+            tab = tune(s)
+        bv, bs = _best(tab["vel"]), _best(tab["str"])
+        print(f"[autotune] {k}: vel {bv[0]}x{bv[1]} {bv[2]:.3f} ms, str {bs[0]}x{bs[1]} {bs[2]:.3f} ms "
+              f"({time.perf_counter()-t0:.1f} s)", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -175,6 +175,9 @@ class Solver:
 
     def __init__(self, case: C.Case, fault: Optional[Dict] = None, fvw_scalars=None, medium=None,
                  newton_max: int = 0, ring_steps: int = 0, init: bool = True):
+        import time
+        clock = time.perf_counter
+        t0 = clock()
         self.L = load()
         self.case = case
         fvw = case.friction == "fvw"
@@ -182,14 +185,17 @@ class Solver:
             fault, fvw_scalars = C.fault_fvw(case)
         self.cfg, self._sta = make_config(case, fvw_scalars, newton_max, ring_steps)
         self.h = ctypes.c_void_p()
+        t1 = clock()
         rc = self.L.fd3d_create(ctypes.byref(self.cfg), ctypes.byref(self.h))
         if rc != 0:
             self.h = None
             raise FD3DError(rc, "fd3d_create failed")
+        t2 = clock()
         if medium is None:
             medium = C.material(case, np.float32)
         lam, mu, d = (_f32(x) for x in medium)
         self._chk(self.L.fd3d_upload_medium(self.h, _fp(lam), _fp(mu), _fp(d)))
+        t3 = clock()
         if fvw:
             arrs = [_f64(fault[n]) for n in C.FVW_FIELDS]
             self._chk(self.L.fd3d_upload_fault_fvw(self.h, *[_dp(a) for a in arrs]))
@@ -198,8 +204,12 @@ class Solver:
                 fault = C.fault_sw(case, np.float32)
             arrs = [_f32(fault[n]) for n in C.SW_FIELDS]
             self._chk(self.L.fd3d_upload_fault_sw(self.h, *[_fp(a) for a in arrs]))
+        t4 = clock()
         if init:
             self.init()
+        # host seconds of each set-up phase
+        self.setup_times = {"inputs": t1 - t0, "create": t2 - t1, "medium": t3 - t2, "fault": t4 - t3,
+                            "init": clock() - t4}
 
     def _chk(self, rc: int):
         if rc != 0:

@@ -375,18 +375,30 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
   return 0;
 }
 
+// 3D fields cross the boundary in the reference's dense layout; on the
+// device every x row is padded to px.  The host packs / unpacks the rows
+// and the transfer is one contiguous copy (a strided 2D copy from pageable
+// memory moves one row at a time).
 static int upload3(fd3d_handle *h, float *dst, const float *src) {
   const Dims &g = h->p.g;
-  HIPCHK(h, hipMemcpy2D(dst + g.off0, (size_t)g.px * sizeof(float), src, (size_t)g.nxt * sizeof(float),
-                        (size_t)g.nxt * sizeof(float), (size_t)g.nyt * g.nzt, hipMemcpyHostToDevice));
+  std::vector<float> &buf = h->staging;
+  buf.assign((size_t)g.n3, 0.0f);
+  const size_t rows = (size_t)g.nyt * g.nzt;
+  for (size_t r = 0; r < rows; ++r)
+    std::memcpy(buf.data() + g.off0 + r * g.px, src + r * g.nxt, (size_t)g.nxt * sizeof(float));
+  HIPCHK(h, hipMemcpy(dst, buf.data(), (size_t)g.n3 * sizeof(float), hipMemcpyHostToDevice));
   return 0;
 }
 
 static int download3(fd3d_handle *h, float *dst, const float *src) {
   const Dims &g = h->p.g;
   HIPCHK(h, owner_sync(h));
-  HIPCHK(h, hipMemcpy2D(dst, (size_t)g.nxt * sizeof(float), src + g.off0, (size_t)g.px * sizeof(float),
-                        (size_t)g.nxt * sizeof(float), (size_t)g.nyt * g.nzt, hipMemcpyDeviceToHost));
+  std::vector<float> &buf = h->staging;
+  buf.resize((size_t)g.n3);
+  HIPCHK(h, hipMemcpy(buf.data(), src, (size_t)g.n3 * sizeof(float), hipMemcpyDeviceToHost));
+  const size_t rows = (size_t)g.nyt * g.nzt;
+  for (size_t r = 0; r < rows; ++r)
+    std::memcpy(dst + r * g.nxt, buf.data() + g.off0 + r * g.px, (size_t)g.nxt * sizeof(float));
   return 0;
 }
 
@@ -407,6 +419,7 @@ int fd3d_upload_medium(fd3d_handle *h, const float *lam1, const float *mu1, cons
   if ((rc = upload3(h, h->p.f.lam, lam1))) return rc;
   if ((rc = upload3(h, h->p.f.mu, mu1))) return rc;
   if ((rc = upload3(h, h->p.f.d, d1))) return rc;
+  std::vector<float>().swap(h->staging);
   h->medium_ok = true;
   return 0;
 }

@@ -25,6 +25,7 @@ struct fd3d_handle {
   std::vector<float> h_striniX, h_striniZ, h_peak, h_dyn, h_Dc, h_coh;
   std::vector<double> h_a, h_b, h_psi, h_vw, h_aX, h_bX, h_psiX, h_vwX;
   int *d_sta = nullptr;
+  std::vector<float> staging;   // padded host copy of one 3D field during transfers
   // profiling
   bool prof = false;
   struct Ev { int id; hipEvent_t a, b; };

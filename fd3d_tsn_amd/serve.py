@@ -29,6 +29,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
+from . import autotune
 from . import capi
 from . import cases as C
 
@@ -123,6 +124,7 @@ class Server:
         self.chunk = chunk
         self.reserve = reserve_bytes
         self.keep_final = keep_final
+        self.tz = tz
         self.batch = capi.Batch(max_slots, tz=tz, mode=mode)
         self.log = log or (lambda m: None)
         self.gpu_busy_s = 0.0
@@ -155,8 +157,9 @@ class Server:
             # admission: a free slot and the footprint in free HBM
             while queue and len(active) + len(ready) < self.max_slots and self._fits(queue[0], 0):
                 r = queue.popleft()
-                ready.append({"req": r, "solver": capi.Solver(r.case), "left": r.steps,
-                              "admitted": clock() - t0})
+                s = capi.Solver(r.case)
+                autotune.apply(s, self.tz)       # rows per tile for this size class
+                ready.append({"req": r, "solver": s, "left": r.steps, "admitted": clock() - t0})
             if queue and not active and not ready:
                 raise RuntimeError(f"request {queue[0].rid} does not fit in device memory")
             if n:
@@ -196,6 +199,7 @@ def run_plain(requests: List[Request], keep_final: bool = True, clock=time.perf_
     for r in sorted(requests, key=lambda q: q.arrival):
         t1 = clock()
         with capi.Solver(r.case) as s:
+            autotune.apply(s)
             s.step(r.steps, outputs=False)
             s.sync()
             fin = s.final() if keep_final else {}
