@@ -15,7 +15,7 @@ import numpy as np
 from . import cases as C
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libfd3d_tsn.so")
+LIB_PATH = os.environ.get("FD3D_TSN_LIB", os.path.join(HERE, "lib", "libfd3d_tsn.so"))
 HEADER = os.path.join(os.path.dirname(HERE), "include", "fd3d_tsn.h")
 
 FD3D_SW, FD3D_FVW = 0, 1

@@ -82,40 +82,42 @@ __device__ __forceinline__ void run_item(const BatchDomain &D, int stage, int b,
   constexpr int BLK = 64 * TZ;
   const Problem &P = D.p;
   const int t = ly * 64 + lx;
+  const FieldPair FP = field_pair(P.f[0], P.f[1], P.ctl);
+  const Fields &fo = FP.fo, &fn = FP.fn;
   switch (stage) {
     case K_VEL_INTERIOR:
-      vel_tiled_body<TZ>(P.g, P.c, P.f, lds, b % D.gx, (b / D.gx) % D.gz, b / (D.gx * D.gz), lx, ly, D.nyv);
+      vel_tiled_body<TZ>(P.g, P.c, fo, fn, lds, b % D.gx, (b / D.gx) % D.gz, b / (D.gx * D.gz), lx, ly, D.nyv);
       break;
     case K_VEL_FAULT:
-      vel_fault_body<TZ, FVW>(P.g, P.c, P.f, P.fl, P.fv, P.ctl, b % D.fpx, b / D.fpx, lx, ly);
+      vel_fault_body<TZ, FVW>(P.g, P.c, fo, fn, P.fl, P.fv, P.ctl, b % D.fpx, b / D.fpx, lx, ly);
       break;
     case K_PML_VEL: {
       const int gi = b * BLK + t;
-      pml_vel_body(P.g, P.c, P.f, P.pml, gi / kPmlBlock, gi % kPmlBlock);
+      pml_vel_body(P.g, P.c, fo, fn, P.pml, gi / kPmlBlock, gi % kPmlBlock);
       break;
     }
     case K_FUVW_UV:
-      fuvw_uv_body<TZ>(P.g, P.f, b % D.fpx, b / D.fpx, lx, ly);
+      fuvw_uv_body<TZ>(P.g, fo, fn, b % D.fpx, b / D.fpx, lx, ly);
       break;
     case K_FUVW_W:
-      fuvw_w_body<TZ>(P.g, P.f, b % D.fpx, b / D.fpx, lx, ly);
+      fuvw_w_body<TZ>(P.g, fo, fn, b % D.fpx, b / D.fpx, lx, ly);
       break;
     case K_STR_INTERIOR:
-      str_tiled_body<TZ>(P.g, P.c, P.f, lds, b % D.gx, (b / D.gx) % D.gz, b / (D.gx * D.gz), lx, ly, D.nys);
+      str_tiled_body<TZ>(P.g, P.c, fo, fn, lds, b % D.gx, (b / D.gx) % D.gz, b / (D.gx * D.gz), lx, ly, D.nys);
       break;
     case K_STR_FAULT:
-      str_fault_body<TZ>(P.g, P.c, P.f, P.fl, b % D.fpx, b / D.fpx, lx, ly);
+      str_fault_body<TZ>(P.g, P.c, fo, fn, P.fl, b % D.fpx, b / D.fpx, lx, ly);
       break;
     case K_PML_STR: {
       const int gi = b * BLK + t;
-      pml_str_body(P.g, P.c, P.f, P.pml, gi / kPmlBlock, gi % kPmlBlock);
+      pml_str_body(P.g, P.c, fo, fn, P.pml, gi / kPmlBlock, gi % kPmlBlock);
       break;
     }
     case K_FRES:
-      fres_body<TZ>(P.g, P.f, b % D.fpx, b / D.fpx, lx, ly);
+      fres_body<TZ>(P.g, fo, fn, b % D.fpx, b / D.fpx, lx, ly);
       break;
     case K_TRACTION:
-      traction_body<TZ>(P.g, P.c, P.f, P.fl, b % D.fix, b / D.fix, lx, ly);
+      traction_body<TZ>(P.g, P.c, fo, fn, P.fl, b % D.fix, b / D.fix, lx, ly);
       break;
     case K_TRACTION_MIRROR:
       traction_mirror_body(P.g, P.fl, b * BLK + t);
@@ -124,13 +126,16 @@ __device__ __forceinline__ void run_item(const BatchDomain &D, int stage, int b,
       tabs_body<TZ>(P.g, P.fl, b % D.fix, b / D.fix, lx, ly);
       break;
     case K_FRICTION:
-      friction_body<TZ, FVW>(P.g, P.c, P.f, P.fl, P.fv, P.ctl, b % D.fix, b / D.fix, lx, ly);
+      friction_body<TZ, FVW>(P.g, P.c, fo, fn, P.fl, P.fv, P.ctl, b % D.fix, b / D.fix, lx, ly);
       break;
     case K_OUTPUTS:
       if (b < D.nout)
-        outputs_body<TZ>(P.g, P.f, P.fl, P.fv, P.out, P.ctl, b % D.fix, b / D.fix, lx, ly);
+        outputs_body<TZ>(P.g, fo, fn, P.fl, P.fv, P.out, P.ctl, b % D.fix, b / D.fix, lx, ly);
       else
-        stations_body(P.g, P.f, P.out, P.ctl, (b - D.nout) * BLK + t);
+        stations_body(P.g, fo, fn, P.out, P.ctl, (b - D.nout) * BLK + t);
+      break;
+    case K_ADVANCE:
+      if (t == 0) advance_body(P.ctl);
       break;
     case K_NUCLEATION:
       nucleation_body<TZ>(P.g, P.c, P.fl, P.fv, P.ctl, b % D.nux, b / D.nux, lx, ly);
@@ -237,7 +242,8 @@ void launch_stage_rt(int s, int n, hipStream_t st, const BatchArgs &A, int lo, i
     case 11: launch_stage<TZ, FVW, 11>(n, st, A, lo, hi, i0); break;
     case 12: launch_stage<TZ, FVW, 12>(n, st, A, lo, hi, i0); break;
     case 13: launch_stage<TZ, FVW, 13>(n, st, A, lo, hi, i0); break;
-    default: launch_stage<TZ, FVW, 14>(n, st, A, lo, hi, i0); break;
+    case 14: launch_stage<TZ, FVW, 14>(n, st, A, lo, hi, i0); break;
+    default: launch_stage<TZ, FVW, 15>(n, st, A, lo, hi, i0); break;
   }
 }
 
@@ -320,6 +326,7 @@ BatchDomain describe(const fd3d_handle *h, int tz) {
   items[K_TRACTION_MIRROR] = cdiv(ni, blk);
   items[K_OUTPUTS] = D.nout + cdiv(h->p.out.nsta, blk);
   items[K_NUCLEATION] = D.nux * cdiv(nzn, tz);
+  items[K_ADVANCE] = 1;
   for (int s = 0; s < K_COUNT; ++s) { D.n[s] = 0; D.dep[s] = -1; }
   int prev = -1;
   for (KernelId id : h->seq) {

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -174,19 +175,30 @@ std::vector<float> build_profiles(const fd3d_config &c, const Dims &g, const Pml
   return out;
 }
 
+// Device allocation of n elements.  skew shifts the array's start by that
+// many bytes (a multiple of 256) inside a slightly larger allocation: the
+// 3D fields are read and written in lock step at equal offsets, and
+// staggering their starts spreads those streams over the HBM channels.
 template <typename T>
-int dalloc(fd3d_handle *h, T **ptr, size_t n) {
+int dalloc(fd3d_handle *h, T **ptr, size_t n, size_t skew = 0) {
   void *p = nullptr;
-  hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+  hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T) + skew);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return fail(h, e == hipErrorOutOfMemory ? FD3D_E_NOMEM : FD3D_E_HIP,
                 std::string("hipMalloc: ") + hipGetErrorString(e));
   }
   h->allocs.push_back(p);
-  h->bytes += n * sizeof(T);
-  *ptr = static_cast<T *>(p);
+  h->bytes += n * sizeof(T) + skew;
+  *ptr = reinterpret_cast<T *>(static_cast<char *>(p) + skew);
   return 0;
+}
+
+// Start offset of the q-th 3D field: odd multiples of 256 bytes modulo 64 KiB.
+size_t field_skew(int q) {
+  static const char *env = std::getenv("FD3D_FIELD_SKEW");
+  const size_t unit = env ? (size_t)std::atol(env) : 2304;
+  return (unit * (size_t)q) % 65536;
 }
 
 int validate(const fd3d_config *c) {
@@ -214,7 +226,7 @@ size_t footprint(const fd3d_config &c) {
   if (build_regions(c, g, pm, nsplit, nprof) != 0) return 0;
   const size_t n2 = (size_t)c.nxt * c.nzt;
   const int ring = c.ring_steps > 0 ? c.ring_steps : 64;
-  size_t b = 12 * (size_t)g.n3 * 4;                 // 3D fields
+  size_t b = 21 * (size_t)g.n3 * 4;                 // 3D fields: two wavefield sets + medium
   b += (size_t)nsplit * 4 + (size_t)nprof * 4;      // PML
   b += n2 * 4 * 33 + n2 * 8 * 8;                    // fault plane
   b += (size_t)ring * FD3D_NOUT * g.nxT * g.nzT * 4 + (size_t)ring * c.nstations * 3 * 4;
@@ -323,10 +335,14 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
   };
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(FD3D_E_HIP);
   const size_t n3 = (size_t)p.g.n3, n2 = (size_t)cfg->nxt * cfg->nzt;
-  float **f3[] = {&p.f.u, &p.f.v, &p.f.w, &p.f.xx, &p.f.yy, &p.f.zz, &p.f.xy, &p.f.yz, &p.f.xz,
-                  &p.f.lam, &p.f.mu, &p.f.d};
-  for (auto pp : f3)
-    if ((rc = dalloc(h, pp, n3))) return bail(rc);
+  // two wavefield sets (the step reads one and writes the other), one medium
+  Fields &A = p.f[0], &B = p.f[1];
+  float **f3[] = {&A.u, &A.v, &A.w, &A.xx, &A.yy, &A.zz, &A.xy, &A.yz, &A.xz,
+                  &B.u, &B.v, &B.w, &B.xx, &B.yy, &B.zz, &B.xy, &B.yz, &B.xz,
+                  &A.lam, &A.mu, &A.d};
+  for (int q = 0; q < (int)(sizeof(f3) / sizeof(f3[0])); ++q)
+    if ((rc = dalloc(h, f3[q], n3, field_skew(q)))) return bail(rc);
+  B.lam = A.lam; B.mu = A.mu; B.d = A.d;
   Fault &fl = p.fl;
   float **f2[] = {&fl.tx, &fl.tz, &fl.v1t, &fl.avdx, &fl.avdz, &fl.RFx, &fl.RFz, &fl.au1, &fl.av1,
                   &fl.aw1, &fl.uZ, &fl.wX, &fl.tabsX, &fl.tabsZ, &fl.T0X, &fl.T0Z, &fl.Dc, &fl.coh,
@@ -370,6 +386,7 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
   if (!p.g.fspace) h->seq.push_back(K_TRACTION_MIRROR);
   h->seq.push_back(K_TABS); h->seq.push_back(K_FRICTION); h->seq.push_back(K_OUTPUTS);
   if (fv.nucleation) h->seq.push_back(K_NUCLEATION);
+  h->seq.push_back(K_ADVANCE);
   if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(FD3D_E_HIP);
   *out = h;
   return 0;
@@ -416,9 +433,9 @@ int fd3d_upload_medium(fd3d_handle *h, const float *lam1, const float *mu1, cons
   const float cfl = vmax * h->cfg.dt / h->cfg.dh;
   if (cfl > 0.25f) return fail(h, FD3D_E_CFL, "numerically unstable: CFL = " + std::to_string(cfl));
   int rc;
-  if ((rc = upload3(h, h->p.f.lam, lam1))) return rc;
-  if ((rc = upload3(h, h->p.f.mu, mu1))) return rc;
-  if ((rc = upload3(h, h->p.f.d, d1))) return rc;
+  if ((rc = upload3(h, h->p.f[0].lam, lam1))) return rc;
+  if ((rc = upload3(h, h->p.f[0].mu, mu1))) return rc;
+  if ((rc = upload3(h, h->p.f[0].d, d1))) return rc;
   std::vector<float>().swap(h->staging);
   h->medium_ok = true;
   return 0;
@@ -468,8 +485,10 @@ int fd3d_init_state(fd3d_handle *h) {
   const int nxt = h->cfg.nxt, nzt = h->cfg.nzt;
   const size_t n2 = (size_t)nxt * nzt;
   hipStream_t s = h->stream;
-  float *f3[] = {p.f.u, p.f.v, p.f.w, p.f.xx, p.f.yy, p.f.zz, p.f.xy, p.f.yz, p.f.xz};
-  for (auto a : f3) HIPCHK(h, hipMemsetAsync(a, 0, n3 * sizeof(float), s));
+  for (const Fields &F : p.f) {
+    float *f3[] = {F.u, F.v, F.w, F.xx, F.yy, F.zz, F.xy, F.yz, F.xz};
+    for (auto a : f3) HIPCHK(h, hipMemsetAsync(a, 0, n3 * sizeof(float), s));
+  }
   Fault &fl = p.fl;
   float *z2[] = {fl.tx, fl.tz, fl.v1t, fl.avdx, fl.avdz, fl.RFx, fl.RFz, fl.au1, fl.av1, fl.aw1, fl.uZ,
                  fl.wX, fl.rise, fl.slipX, fl.slipZ, fl.schangeX, fl.schangeZ, fl.distX, fl.distZ};
@@ -621,8 +640,10 @@ int fd3d_download_final(fd3d_handle *h, float *out7) {
   return 0;
 }
 
-static float *field_ptr(fd3d_handle *h, int which) {
-  Fields &f = h->p.f;
+// Field `which` of wavefield set `set` (the state after the last step is
+// in set steps & 1).
+static float *field_ptr(fd3d_handle *h, int which, int set) {
+  Fields &f = h->p.f[set];
   switch (which) {
     case FD3D_U: return f.u; case FD3D_V: return f.v; case FD3D_W: return f.w;
     case FD3D_XX: return f.xx; case FD3D_YY: return f.yy; case FD3D_ZZ: return f.zz;
@@ -633,17 +654,20 @@ static float *field_ptr(fd3d_handle *h, int which) {
 
 int fd3d_download_field(fd3d_handle *h, int which, float *out) {
   if (!h || !out) return fail(h, FD3D_E_INVALID, "null argument");
-  float *src = field_ptr(h, which);
+  float *src = field_ptr(h, which, h->steps & 1);
   if (!src) return fail(h, FD3D_E_INVALID, "unknown field");
   return download3(h, out, src);
 }
 
 int fd3d_upload_field(fd3d_handle *h, int which, const float *in) {
   if (!h || !in) return fail(h, FD3D_E_INVALID, "null argument");
-  float *dst = field_ptr(h, which);
-  if (!dst) return fail(h, FD3D_E_INVALID, "unknown field");
+  if (!field_ptr(h, which, 0)) return fail(h, FD3D_E_INVALID, "unknown field");
   HIPCHK(h, owner_sync(h));
-  return upload3(h, dst, in);
+  // both sets: cells no stage ever writes (the outer faces) must read the
+  // same in either set
+  int rc = upload3(h, field_ptr(h, which, 0), in);
+  if (rc == 0) rc = upload3(h, field_ptr(h, which, 1), in);
+  return rc;
 }
 
 int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str) {

@@ -83,7 +83,7 @@ struct Pml {
 };
 
 struct Ctl {
-  int it;              // steps taken since init (time = it*dt)
+  int it;              // steps completed since init; the state after step n is in field set n & 1
   int newton_capped;   // Newton solves that hit the cap
   int nonfinite;       // NaN/Inf seen in the slip-rate outputs
   int pad;
@@ -110,7 +110,7 @@ struct Problem {
   Tuning tune;
   Dims g;
   Coef c;
-  Fields f;
+  Fields f[2];         // the two wavefield sets (u..xz); lam, mu, d shared
   Fault fl;
   FvwScalars fv;
   Pml pml;
@@ -121,7 +121,7 @@ struct Problem {
 enum KernelId {
   K_VEL_INTERIOR = 0, K_VEL_FAULT, K_PML_VEL, K_FUVW_UV, K_FUVW_W,
   K_STR_INTERIOR, K_STR_FAULT, K_PML_STR, K_FRES,
-  K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_NUCLEATION,
+  K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_NUCLEATION, K_ADVANCE,
   K_COUNT
 };
 extern const char *kKernelNames[K_COUNT];

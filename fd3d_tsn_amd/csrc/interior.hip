@@ -2,23 +2,28 @@
 // are numbered so that each XCD (block id modulo 8 on MI355X) receives a
 // contiguous run of the logical tile order, keeping x and z neighbour
 // tiles on the same L2.
+#include "stages.h"
 #include "tiles.h"
 
 namespace fd3d {
 namespace {
 
 template <int TZ>
-__global__ void __launch_bounds__(64 * TZ) k_vel_tiled(Dims g, Coef c, Fields f, int gx, int gz, int gy, int NY) {
+__global__ void __launch_bounds__(64 * TZ) k_vel_tiled(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl, int gx,
+                                                       int gz, int gy, int NY) {
   __shared__ float lds[Tile<TZ>::LDS_FLOATS];
   const int lid = xcd_remap(blockIdx.x, gx * gz * gy);
-  vel_tiled_body<TZ>(g, c, f, lds, lid % gx, (lid / gx) % gz, lid / (gx * gz), threadIdx.x, threadIdx.y, NY);
+  const FieldPair FP = field_pair(f0, f1, ctl);
+  vel_tiled_body<TZ>(g, c, FP.fo, FP.fn, lds, lid % gx, (lid / gx) % gz, lid / (gx * gz), threadIdx.x, threadIdx.y, NY);
 }
 
 template <int TZ>
-__global__ void __launch_bounds__(64 * TZ) k_str_tiled(Dims g, Coef c, Fields f, int gx, int gz, int gy, int NY) {
+__global__ void __launch_bounds__(64 * TZ) k_str_tiled(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl, int gx,
+                                                       int gz, int gy, int NY) {
   __shared__ float lds[2 * 4 * Tile<TZ>::N];
   const int lid = xcd_remap(blockIdx.x, gx * gz * gy);
-  str_tiled_body<TZ>(g, c, f, lds, lid % gx, (lid / gx) % gz, lid / (gx * gz), threadIdx.x, threadIdx.y, NY);
+  const FieldPair FP = field_pair(f0, f1, ctl);
+  str_tiled_body<TZ>(g, c, FP.fo, FP.fn, lds, lid % gx, (lid / gx) % gz, lid / (gx * gz), threadIdx.x, threadIdx.y, NY);
 }
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -28,7 +33,8 @@ hipError_t launch_vel(const Problem &p, hipStream_t s, int ny) {
   const Dims &g = p.g;
   const int gx = cdiv(g.nxt - 2 * g.nabc, 64), gz = cdiv(g.nzt - g.nfs - g.nabc, TZ);
   const int gy = cdiv(g.nyt - 2 - g.nabc, ny);
-  hipLaunchKernelGGL(k_vel_tiled<TZ>, dim3(gx * gz * gy), dim3(64, TZ), 0, s, g, p.c, p.f, gx, gz, gy, ny);
+  hipLaunchKernelGGL(k_vel_tiled<TZ>, dim3(gx * gz * gy), dim3(64, TZ), 0, s, g, p.c, p.f[0], p.f[1], p.ctl, gx, gz,
+                     gy, ny);
   return hipGetLastError();
 }
 
@@ -37,7 +43,8 @@ hipError_t launch_str(const Problem &p, hipStream_t s, int ny) {
   const Dims &g = p.g;
   const int gx = cdiv(g.nxt - 2 * g.nabc, 64), gz = cdiv(g.nzt - g.nfs - g.nabc, TZ);
   const int gy = cdiv(g.nyt - 2 - g.nabc, ny);
-  hipLaunchKernelGGL(k_str_tiled<TZ>, dim3(gx * gz * gy), dim3(64, TZ), 0, s, g, p.c, p.f, gx, gz, gy, ny);
+  hipLaunchKernelGGL(k_str_tiled<TZ>, dim3(gx * gz * gy), dim3(64, TZ), 0, s, g, p.c, p.f[0], p.f[1], p.ctl, gx, gz,
+                     gy, ny);
   return hipGetLastError();
 }
 

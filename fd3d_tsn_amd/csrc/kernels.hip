@@ -5,6 +5,7 @@
 //   velocity : vel_interior | vel_fault | pml_vel | (fuvw_uv, fuvw_w)
 //   stress   : str_interior | str_fault | pml_str | (fres)
 //   fault BC : traction | (traction_mirror) | tabs | friction | outputs | (nucleation)
+//   advance  : the step counter (which also selects the wavefield set)
 // Kernels in parentheses run only with a free surface / rate-and-state
 // nucleation.  vel_fault performs the v mirror v(:,nyt,:) = v(:,nyt-1,:)
 // for every column before pml_vel updates the PML columns, as bnd2d does
@@ -17,58 +18,83 @@ namespace fd3d {
 const char *kKernelNames[K_COUNT] = {
     "vel_interior", "vel_fault", "pml_vel", "fuvw_uv", "fuvw_w",
     "str_interior", "str_fault", "pml_str", "fres",
-    "traction", "traction_mirror", "tabs", "friction", "outputs", "nucleation"};
+    "traction", "traction_mirror", "tabs", "friction", "outputs", "nucleation", "advance"};
 
 namespace {
 
 constexpr int BY = 4;  // lane rows of the 64 x 4 blocks of the 2D stages
 
 #define B2 blockIdx.x, blockIdx.y, threadIdx.x, threadIdx.y
+#define PAIR                                     \
+  const FieldPair FP = field_pair(f0, f1, ctl); \
+  const Fields &fo = FP.fo, &fn = FP.fn
 
-__global__ void __launch_bounds__(256) k_vel_interior(Dims g, Coef c, Fields f) {
-  vel_interior_body(g, c, f, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x, threadIdx.y);
+__global__ void __launch_bounds__(256) k_vel_interior(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl) {
+  PAIR;
+  vel_interior_body(g, c, fo, fn, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x, threadIdx.y);
 }
-__global__ void __launch_bounds__(256) k_vel_fault(Dims g, Coef c, Fields f, Fault fl, FvwScalars fv, Ctl *ctl) {
-  vel_fault_body<BY>(g, c, f, fl, fv, ctl, B2);
+__global__ void __launch_bounds__(256) k_vel_fault(Dims g, Coef c, Fields f0, Fields f1, Fault fl, FvwScalars fv,
+                                                   Ctl *ctl) {
+  PAIR;
+  vel_fault_body<BY>(g, c, fo, fn, fl, fv, ctl, B2);
 }
-__global__ void __launch_bounds__(kPmlBlock) k_pml_vel(Dims g, Coef c, Fields f, Pml pm) {
-  pml_vel_body(g, c, f, pm, blockIdx.x, threadIdx.x);
+__global__ void __launch_bounds__(kPmlBlock) k_pml_vel(Dims g, Coef c, Fields f0, Fields f1, Pml pm, const Ctl *ctl) {
+  PAIR;
+  pml_vel_body(g, c, fo, fn, pm, blockIdx.x, threadIdx.x);
 }
-__global__ void __launch_bounds__(256) k_fuvw_uv(Dims g, Fields f) { fuvw_uv_body<BY>(g, f, B2); }
-__global__ void __launch_bounds__(256) k_fuvw_w(Dims g, Fields f) { fuvw_w_body<BY>(g, f, B2); }
-__global__ void __launch_bounds__(256) k_str_interior(Dims g, Coef c, Fields f) {
-  str_interior_body(g, c, f, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x, threadIdx.y);
+__global__ void __launch_bounds__(256) k_fuvw_uv(Dims g, Fields f0, Fields f1, const Ctl *ctl) {
+  PAIR;
+  fuvw_uv_body<BY>(g, fo, fn, B2);
 }
-__global__ void __launch_bounds__(256) k_str_fault(Dims g, Coef c, Fields f, Fault fl) {
-  str_fault_body<BY>(g, c, f, fl, B2);
+__global__ void __launch_bounds__(256) k_fuvw_w(Dims g, Fields f0, Fields f1, const Ctl *ctl) {
+  PAIR;
+  fuvw_w_body<BY>(g, fo, fn, B2);
 }
-__global__ void __launch_bounds__(kPmlBlock) k_pml_str(Dims g, Coef c, Fields f, Pml pm) {
-  pml_str_body(g, c, f, pm, blockIdx.x, threadIdx.x);
+__global__ void __launch_bounds__(256) k_str_interior(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl) {
+  PAIR;
+  str_interior_body(g, c, fo, fn, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x, threadIdx.y);
 }
-__global__ void __launch_bounds__(256) k_fres(Dims g, Fields f) { fres_body<BY>(g, f, B2); }
-__global__ void __launch_bounds__(256) k_traction(Dims g, Coef c, Fields f, Fault fl) {
-  traction_body<BY>(g, c, f, fl, B2);
+__global__ void __launch_bounds__(256) k_str_fault(Dims g, Coef c, Fields f0, Fields f1, Fault fl, const Ctl *ctl) {
+  PAIR;
+  str_fault_body<BY>(g, c, fo, fn, fl, B2);
+}
+__global__ void __launch_bounds__(kPmlBlock) k_pml_str(Dims g, Coef c, Fields f0, Fields f1, Pml pm, const Ctl *ctl) {
+  PAIR;
+  pml_str_body(g, c, fo, fn, pm, blockIdx.x, threadIdx.x);
+}
+__global__ void __launch_bounds__(256) k_fres(Dims g, Fields f0, Fields f1, const Ctl *ctl) {
+  PAIR;
+  fres_body<BY>(g, fo, fn, B2);
+}
+__global__ void __launch_bounds__(256) k_traction(Dims g, Coef c, Fields f0, Fields f1, Fault fl, const Ctl *ctl) {
+  PAIR;
+  traction_body<BY>(g, c, fo, fn, fl, B2);
 }
 __global__ void __launch_bounds__(256) k_traction_mirror(Dims g, Fault fl) {
   traction_mirror_body(g, fl, blockIdx.x * 256 + threadIdx.x);
 }
 __global__ void __launch_bounds__(256) k_tabs(Dims g, Fault fl) { tabs_body<BY>(g, fl, B2); }
-__global__ void __launch_bounds__(256) k_friction(Dims g, Coef c, Fields f, Fault fl, FvwScalars fv,
+__global__ void __launch_bounds__(256) k_friction(Dims g, Coef c, Fields f0, Fields f1, Fault fl, FvwScalars fv,
                                                   const Ctl *ctl) {
-  friction_body<BY>(g, c, f, fl, fv, ctl, B2);
+  PAIR;
+  friction_body<BY>(g, c, fo, fn, fl, fv, ctl, B2);
 }
 // The last block row records the stations.
-__global__ void __launch_bounds__(256) k_outputs(Dims g, Fields f, Fault fl, FvwScalars fv, Outputs o, Ctl *ctl) {
+__global__ void __launch_bounds__(256) k_outputs(Dims g, Fields f0, Fields f1, Fault fl, FvwScalars fv, Outputs o,
+                                                 Ctl *ctl) {
+  PAIR;
   if (blockIdx.y == gridDim.y - 1) {
-    stations_body(g, f, o, ctl, blockIdx.x * 256 + threadIdx.y * BX + threadIdx.x);
+    stations_body(g, fo, fn, o, ctl, blockIdx.x * 256 + threadIdx.y * BX + threadIdx.x);
     return;
   }
-  outputs_body<BY>(g, f, fl, fv, o, ctl, B2);
+  outputs_body<BY>(g, fo, fn, fl, fv, o, ctl, B2);
 }
 __global__ void __launch_bounds__(256) k_nucleation(Dims g, Coef c, Fault fl, FvwScalars fv, const Ctl *ctl) {
   nucleation_body<BY>(g, c, fl, fv, ctl, B2);
 }
+__global__ void k_advance(Ctl *ctl) { advance_body(ctl); }
 
+#undef PAIR
 #undef B2
 
 __global__ void k_fill(float *p, float v, long long n) {
@@ -99,41 +125,42 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
     case K_VEL_INTERIOR:
       if (nyi <= 0) return hipSuccess;
       if (p.tune.mode & 1) return launch_interior(false, p, s);
-      hipLaunchKernelGGL(k_vel_interior, dim3(cdiv(ni, BX), cdiv(nyi, BY), nk), b2, 0, s, g, p.c, p.f);
+      hipLaunchKernelGGL(k_vel_interior, dim3(cdiv(ni, BX), cdiv(nyi, BY), nk), b2, 0, s, g, p.c, p.f[0], p.f[1], p.ctl);
       break;
     case K_VEL_FAULT:
-      hipLaunchKernelGGL(k_vel_fault, dim3(cdiv(g.nxt, BX), cdiv(g.nzt, BY)), b2, 0, s, g, p.c, p.f, p.fl,
-                         p.fv, p.ctl);
+      hipLaunchKernelGGL(k_vel_fault, dim3(cdiv(g.nxt, BX), cdiv(g.nzt, BY)), b2, 0, s, g, p.c, p.f[0], p.f[1],
+                         p.fl, p.fv, p.ctl);
       break;
     case K_PML_VEL:
       if (p.pml.nblk == 0) return hipSuccess;
       if (p.tune.mode & 2) return launch_pml_zm(false, p, s);
-      hipLaunchKernelGGL(k_pml_vel, dim3(p.pml.nblk), dim3(kPmlBlock), 0, s, g, p.c, p.f, p.pml);
+      hipLaunchKernelGGL(k_pml_vel, dim3(p.pml.nblk), dim3(kPmlBlock), 0, s, g, p.c, p.f[0], p.f[1], p.pml, p.ctl);
       break;
     case K_FUVW_UV:
-      hipLaunchKernelGGL(k_fuvw_uv, dim3(cdiv(g.nxt, BX), cdiv(g.nyt, BY)), b2, 0, s, g, p.f);
+      hipLaunchKernelGGL(k_fuvw_uv, dim3(cdiv(g.nxt, BX), cdiv(g.nyt, BY)), b2, 0, s, g, p.f[0], p.f[1], p.ctl);
       break;
     case K_FUVW_W:
-      hipLaunchKernelGGL(k_fuvw_w, dim3(cdiv(g.nxt, BX), cdiv(g.nyt, BY)), b2, 0, s, g, p.f);
+      hipLaunchKernelGGL(k_fuvw_w, dim3(cdiv(g.nxt, BX), cdiv(g.nyt, BY)), b2, 0, s, g, p.f[0], p.f[1], p.ctl);
       break;
     case K_STR_INTERIOR:
       if (nyi <= 0) return hipSuccess;
       if (p.tune.mode & 1) return launch_interior(true, p, s);
-      hipLaunchKernelGGL(k_str_interior, dim3(cdiv(ni, BX), cdiv(nyi, BY), nk), b2, 0, s, g, p.c, p.f);
+      hipLaunchKernelGGL(k_str_interior, dim3(cdiv(ni, BX), cdiv(nyi, BY), nk), b2, 0, s, g, p.c, p.f[0], p.f[1], p.ctl);
       break;
     case K_STR_FAULT:
-      hipLaunchKernelGGL(k_str_fault, dim3(cdiv(g.nxt, BX), cdiv(g.nzt, BY)), b2, 0, s, g, p.c, p.f, p.fl);
+      hipLaunchKernelGGL(k_str_fault, dim3(cdiv(g.nxt, BX), cdiv(g.nzt, BY)), b2, 0, s, g, p.c, p.f[0], p.f[1], p.fl,
+                         p.ctl);
       break;
     case K_PML_STR:
       if (p.pml.nblk == 0) return hipSuccess;
       if (p.tune.mode & 2) return launch_pml_zm(true, p, s);
-      hipLaunchKernelGGL(k_pml_str, dim3(p.pml.nblk), dim3(kPmlBlock), 0, s, g, p.c, p.f, p.pml);
+      hipLaunchKernelGGL(k_pml_str, dim3(p.pml.nblk), dim3(kPmlBlock), 0, s, g, p.c, p.f[0], p.f[1], p.pml, p.ctl);
       break;
     case K_FRES:
-      hipLaunchKernelGGL(k_fres, dim3(cdiv(g.nxt, BX), cdiv(g.nyt, BY)), b2, 0, s, g, p.f);
+      hipLaunchKernelGGL(k_fres, dim3(cdiv(g.nxt, BX), cdiv(g.nyt, BY)), b2, 0, s, g, p.f[0], p.f[1], p.ctl);
       break;
     case K_TRACTION:
-      hipLaunchKernelGGL(k_traction, dim3(cdiv(ni, BX), cdiv(nk, BY)), b2, 0, s, g, p.c, p.f, p.fl);
+      hipLaunchKernelGGL(k_traction, dim3(cdiv(ni, BX), cdiv(nk, BY)), b2, 0, s, g, p.c, p.f[0], p.f[1], p.fl, p.ctl);
       break;
     case K_TRACTION_MIRROR:
       hipLaunchKernelGGL(k_traction_mirror, dim3(cdiv(ni, 256)), dim3(256), 0, s, g, p.fl);
@@ -142,14 +169,15 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
       hipLaunchKernelGGL(k_tabs, dim3(cdiv(ni, BX), cdiv(nk, BY)), b2, 0, s, g, p.fl);
       break;
     case K_FRICTION:
-      hipLaunchKernelGGL(k_friction, dim3(cdiv(ni, BX), cdiv(nk, BY)), b2, 0, s, g, p.c, p.f, p.fl, p.fv,
-                         p.ctl);
+      hipLaunchKernelGGL(k_friction, dim3(cdiv(ni, BX), cdiv(nk, BY)), b2, 0, s, g, p.c, p.f[0], p.f[1], p.fl,
+                         p.fv, p.ctl);
       break;
     case K_OUTPUTS: {
       unsigned gx = cdiv(ni, BX);
       unsigned sta_blocks = cdiv(p.out.nsta, BX * BY);
       if (sta_blocks > gx) gx = sta_blocks;
-      hipLaunchKernelGGL(k_outputs, dim3(gx, cdiv(nk, BY) + 1), b2, 0, s, g, p.f, p.fl, p.fv, p.out, p.ctl);
+      hipLaunchKernelGGL(k_outputs, dim3(gx, cdiv(nk, BY) + 1), b2, 0, s, g, p.f[0], p.f[1], p.fl, p.fv, p.out,
+                         p.ctl);
       break;
     }
     case K_NUCLEATION: {
@@ -158,6 +186,9 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
       hipLaunchKernelGGL(k_nucleation, dim3(cdiv(nxn, BX), cdiv(nzn, BY)), b2, 0, s, g, p.c, p.fl, p.fv, p.ctl);
       break;
     }
+    case K_ADVANCE:
+      hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, p.ctl);
+      break;
     default:
       return hipErrorInvalidValue;
   }

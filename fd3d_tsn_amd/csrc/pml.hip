@@ -10,7 +10,7 @@
 // are read and written in contiguous runs.  All five regions share one
 // launch; a block finds its region from a prefix table.
 // The per-point arithmetic is that of k_pml_vel / k_pml_str in kernels.hip.
-#include "fd3d_dev.h"
+#include "stages.h"
 
 namespace fd3d {
 namespace {
@@ -36,7 +36,10 @@ __device__ inline bool pml_zcell(const Pml &pm, RegionCell &c) {
   return true;
 }
 
-__global__ void __launch_bounds__(kPmlBlock) k_pml_vel_zm(Dims g, Coef c, Fields f, Pml pm) {
+__global__ void __launch_bounds__(kPmlBlock) k_pml_vel_zm(Dims g, Coef c, Fields f0, Fields f1, Pml pm,
+                                                                const Ctl *ctl) {
+  const FieldPair FP = field_pair(f0, f1, ctl);
+  const Fields &fo = FP.fo, &fn = FP.fn;
   RegionCell rc;
   if (!pml_zcell(pm, rc)) return;
   const PmlRegion &R = pm.r[rc.r];
@@ -54,43 +57,46 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_vel_zm(Dims g, Coef c, Fields
   float *s = pm.splits + R.split_off + (i2 - 1) + (long long)R.ni * (j2 - 1) + nxy * (k2b - 1);
   const float dt = c.dt, dh = c.dh;
   int q = I3(g, i, j, R.kb + k2b - 1);
-  float xzm1 = f.xz[q - sz], xz0 = f.xz[q];
-  float zz0 = f.zz[q], zzp1 = f.zz[q + sz];
-  float yzm1 = f.yz[q - sz], yz0 = f.yz[q];
+  float xzm1 = fo.xz[q - sz], xz0 = fo.xz[q];
+  float zz0 = fo.zz[q], zzp1 = fo.zz[q + sz];
+  float yzm1 = fo.yz[q - sz], yz0 = fo.yz[q];
   for (int k2 = k2b;; ++k2) {
     const float az = amz[k2], bz = apz[k2];
-    const float dtd = dt / f.d[q];
+    const float dtd = dt / fo.d[q];
     float pt, s1, s2, s3;
     s1 = s[0]; s2 = s[nc]; s3 = s[2 * nc];
-    pt = s1 * amx + (dtd * (f.xx[q] - f.xx[q - 1])) / dh; s1 = pt / apx;
-    pt = s2 * amy + (dtd * (f.xy[q] - f.xy[q - sy])) / dh; s2 = pt / apy;
+    pt = s1 * amx + (dtd * (fo.xx[q] - fo.xx[q - 1])) / dh; s1 = pt / apx;
+    pt = s2 * amy + (dtd * (fo.xy[q] - fo.xy[q - sy])) / dh; s2 = pt / apy;
     pt = s3 * az + (dtd * (xz0 - xzm1)) / dh; s3 = pt / bz;
     s[0] = s1; s[nc] = s2; s[2 * nc] = s3;
-    f.u[q] = (s1 + s2) + s3;
+    fn.u[q] = (s1 + s2) + s3;
     s1 = s[6 * nc]; s2 = s[7 * nc]; s3 = s[8 * nc];
-    pt = s1 * amxS + (dtd * (f.xz[q + 1] - xz0)) / dh; s1 = pt / apxS;
-    pt = s2 * amy + (dtd * (yz0 - f.yz[q - sy])) / dh; s2 = pt / apy;
+    pt = s1 * amxS + (dtd * (fo.xz[q + 1] - xz0)) / dh; s1 = pt / apxS;
+    pt = s2 * amy + (dtd * (yz0 - fo.yz[q - sy])) / dh; s2 = pt / apy;
     pt = s3 * az + (dtd * (zzp1 - zz0)) / dh; s3 = pt / bz;
     s[6 * nc] = s1; s[7 * nc] = s2; s[8 * nc] = s3;
-    f.w[q] = (s1 + s2) + s3;
+    fn.w[q] = (s1 + s2) + s3;
     if (has_v) {
       s1 = s[3 * nc]; s2 = s[4 * nc]; s3 = s[5 * nc];
-      pt = s1 * amxS + (dtd * (f.xy[q + 1] - f.xy[q])) / dh; s1 = pt / apxS;
-      pt = s2 * amy + (dtd * (f.yy[q + sy] - f.yy[q])) / dh; s2 = pt / apy;
+      pt = s1 * amxS + (dtd * (fo.xy[q + 1] - fo.xy[q])) / dh; s1 = pt / apxS;
+      pt = s2 * amy + (dtd * (fo.yy[q + sy] - fo.yy[q])) / dh; s2 = pt / apy;
       pt = s3 * az + (dtd * (yz0 - yzm1)) / dh; s3 = pt / bz;
       s[3 * nc] = s1; s[4 * nc] = s2; s[5 * nc] = s3;
-      f.v[q] = (s1 + s2) + s3;
+      fn.v[q] = (s1 + s2) + s3;
     }
     if (k2 == k2e) break;
     q += sz;
     s += nxy;
-    xzm1 = xz0; xz0 = f.xz[q];
-    zz0 = zzp1; zzp1 = f.zz[q + sz];
-    yzm1 = yz0; yz0 = f.yz[q];
+    xzm1 = xz0; xz0 = fo.xz[q];
+    zz0 = zzp1; zzp1 = fo.zz[q + sz];
+    yzm1 = yz0; yz0 = fo.yz[q];
   }
 }
 
-__global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields f, Pml pm) {
+__global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields f0, Fields f1, Pml pm,
+                                                                const Ctl *ctl) {
+  const FieldPair FP = field_pair(f0, f1, ctl);
+  const Fields &fo = FP.fo, &fn = FP.fn;
   RegionCell rc;
   if (!pml_zcell(pm, rc)) return;
   const PmlRegion &R = pm.r[rc.r];
@@ -108,13 +114,13 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields
   const long long nc = R.ncell, nxy = (long long)R.ni * R.nj;
   float *s = pm.splits + R.split_off + (i2 - 1) + (long long)R.ni * (j2 - 1) + nxy * (k2b - 1);
   const float dt = c.dt, dh = c.dh;
-  const float *u = f.u, *v = f.v, *w = f.w, *mu = f.mu;
+  const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fo.mu;
   int q = I3(g, i, j, R.kb + k2b - 1);
   float u0 = u[q], up1 = u[q + sz], v0 = v[q], vp1 = v[q + sz];
   float wm1 = w[q - sz], w0 = w[q], m0 = mu[q], mp1 = mu[q + sz];
   for (int k2 = k2b;; ++k2) {
     const float az = amz[k2], bz = apz[k2];
-    const float xl = f.lam[q], xm = m0;
+    const float xl = fo.lam[q], xm = m0;
     const float a = xl + 2.0f * xm, b = xl;
     const float diff1 = u[q + 1] - u0, diff2 = v0 - v[q - sy], diff3 = w0 - wm1;
     float pt, s1, s2, s3;
@@ -123,19 +129,19 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields
     pt = s2 * ay + ((dt * b) * diff2) / dh; s2 = pt / by;
     pt = s3 * az + ((dt * b) * diff3) / dh; s3 = pt / bz;
     s[9 * nc] = s1; s[10 * nc] = s2; s[11 * nc] = s3;
-    f.xx[q] = (s1 + s2) + s3;
+    fn.xx[q] = (s1 + s2) + s3;
     s1 = s[12 * nc]; s2 = s[13 * nc]; s3 = s[14 * nc];
     pt = s1 * ax + ((dt * b) * diff1) / dh; s1 = pt / bx;
     pt = s2 * ay + ((dt * a) * diff2) / dh; s2 = pt / by;
     pt = s3 * az + ((dt * b) * diff3) / dh; s3 = pt / bz;
     s[12 * nc] = s1; s[13 * nc] = s2; s[14 * nc] = s3;
-    f.yy[q] = (s1 + s2) + s3;
+    fn.yy[q] = (s1 + s2) + s3;
     s1 = s[15 * nc]; s2 = s[16 * nc]; s3 = s[17 * nc];
     pt = s1 * ax + ((dt * b) * diff1) / dh; s1 = pt / bx;
     pt = s2 * ay + ((dt * b) * diff2) / dh; s2 = pt / by;
     pt = s3 * az + ((dt * a) * diff3) / dh; s3 = pt / bz;
     s[15 * nc] = s1; s[16 * nc] = s2; s[17 * nc] = s3;
-    f.zz[q] = (s1 + s2) + s3;
+    fn.zz[q] = (s1 + s2) + s3;
     const bool zero_top = R.ztop && k2 == R.nk;
     if (has_v) {
       const float xmu = (xm + mu[q + sy]) / 2.0f;
@@ -143,14 +149,14 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields
       pt = s1 * amyS + ((dt * xmu) * (u[q + sy] - u0)) / dh; s1 = pt / apyS;
       pt = s2 * amx + ((dt * xmu) * (v0 - v[q - 1])) / dh; s2 = pt / apx;
       s[18 * nc] = s1; s[19 * nc] = s2;
-      f.xy[q] = s1 + s2;
+      fn.xy[q] = s1 + s2;
     }
     if (has_xz) {
       const float xmu = (xm + mp1) / 2.0f;
       s1 = s[20 * nc]; s2 = s[21 * nc];
       pt = s1 * amzS[k2] + ((dt * xmu) * (up1 - u0)) / dh; s1 = pt / apzS[k2];
       pt = s2 * amx + ((dt * xmu) * (w0 - w[q - 1])) / dh; s2 = pt / apx;
-      f.xz[q] = s1 + s2;
+      fn.xz[q] = s1 + s2;
       if (zero_top && j2 <= R.nzx) { s1 = 0.0f; s2 = 0.0f; }
       s[20 * nc] = s1; s[21 * nc] = s2;
     }
@@ -159,7 +165,7 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields
       s1 = s[22 * nc]; s2 = s[23 * nc];
       pt = s1 * amzS[k2] + ((dt * xmu) * (vp1 - v0)) / dh; s1 = pt / apzS[k2];
       pt = s2 * amyS + ((dt * xmu) * (w[q + sy] - w0)) / dh; s2 = pt / apyS;
-      f.yz[q] = s1 + s2;
+      fn.yz[q] = s1 + s2;
       if (zero_top && j2 <= R.nzy) { s1 = 0.0f; s2 = 0.0f; }
       s[22 * nc] = s1; s[23 * nc] = s2;
     }
@@ -178,9 +184,11 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields
 hipError_t launch_pml_zm(bool stress, const Problem &p, hipStream_t s) {
   if (p.pml.nblkz == 0) return hipSuccess;
   if (stress)
-    hipLaunchKernelGGL(k_pml_str_zm, dim3(p.pml.nblkz), dim3(kPmlBlock), 0, s, p.g, p.c, p.f, p.pml);
+    hipLaunchKernelGGL(k_pml_str_zm, dim3(p.pml.nblkz), dim3(kPmlBlock), 0, s, p.g, p.c, p.f[0], p.f[1], p.pml,
+                       p.ctl);
   else
-    hipLaunchKernelGGL(k_pml_vel_zm, dim3(p.pml.nblkz), dim3(kPmlBlock), 0, s, p.g, p.c, p.f, p.pml);
+    hipLaunchKernelGGL(k_pml_vel_zm, dim3(p.pml.nblkz), dim3(kPmlBlock), 0, s, p.g, p.c, p.f[0], p.f[1], p.pml,
+                       p.ctl);
   return hipGetLastError();
 }
 

@@ -7,6 +7,14 @@
 // items from a table.  2D stages are templated on the number of lane rows
 // BY of a 64 x BY block; results do not depend on BY.
 //
+// Wavefield buffers.  u, v, w and the six stresses exist twice; a step
+// reads the state of the previous step from one set (fo) and writes the
+// new state into the other (fn), so no stage overwrites a value another
+// stage of the same step still has to read, whatever the order of blocks.
+// Which set is which follows from the parity of the step counter
+// Ctl::it, which only the final stage of a step (advance_body) changes:
+// the state after step n is in set n & 1.  lam, mu and d are shared.
+//
 // Arithmetic contract: every update evaluates the reference's expression
 // with the same operands in the same left-to-right association, built with
 // -ffp-contract=off (no fused multiply-add), IEEE-correct float division
@@ -31,22 +39,22 @@ __device__ inline double sqd(double x) { return x * x; }
 // ---------------------------------------------------------------------------
 constexpr int BYF = 4;  // rows per block of the one-thread-per-cell interior sweeps
 
-__device__ inline void vel_interior_body(const Dims &g, const Coef &c, const Fields &f, int gbx, int gby, int gbz, int lx, int ly) {
+__device__ inline void vel_interior_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, int gbx, int gby, int gbz, int lx, int ly) {
   const int i = g.nabc + 1 + gbx * BX + lx;
   const int j = g.nabc + 1 + gby * BYF + ly;
   const int k = g.nabc + 1 + gbz;
   if (i > g.nxt - g.nabc || j > g.nyt - 2) return;
   const int q = I3(g, i, j, k), sy = g.sy, sz = g.sz;
   const float c1 = c.c1, c2 = c.c2;
-  const float b = c.dth / f.d[q];
-  const float *xx = f.xx, *yy = f.yy, *zz = f.zz, *xy = f.xy, *yz = f.yz, *xz = f.xz;
-  f.u[q] = f.u[q] + b * (c1 * (xx[q] - xx[q - 1]) + c2 * (xx[q + 1] - xx[q - 2]) +
+  const float b = c.dth / fo.d[q];
+  const float *xx = fo.xx, *yy = fo.yy, *zz = fo.zz, *xy = fo.xy, *yz = fo.yz, *xz = fo.xz;
+  fn.u[q] = fo.u[q] + b * (c1 * (xx[q] - xx[q - 1]) + c2 * (xx[q + 1] - xx[q - 2]) +
                          c1 * (xy[q] - xy[q - sy]) + c2 * (xy[q + sy] - xy[q - 2 * sy]) +
                          c1 * (xz[q] - xz[q - sz]) + c2 * (xz[q + sz] - xz[q - 2 * sz]));
-  f.v[q] = f.v[q] + b * (c1 * (xy[q + 1] - xy[q]) + c2 * (xy[q + 2] - xy[q - 1]) +
+  fn.v[q] = fo.v[q] + b * (c1 * (xy[q + 1] - xy[q]) + c2 * (xy[q + 2] - xy[q - 1]) +
                          c1 * (yy[q + sy] - yy[q]) + c2 * (yy[q + 2 * sy] - yy[q - sy]) +
                          c1 * (yz[q] - yz[q - sz]) + c2 * (yz[q + sz] - yz[q - 2 * sz]));
-  f.w[q] = f.w[q] + b * (c1 * (xz[q + 1] - xz[q]) + c2 * (xz[q + 2] - xz[q - 1]) +
+  fn.w[q] = fo.w[q] + b * (c1 * (xz[q + 1] - xz[q]) + c2 * (xz[q + 2] - xz[q - 1]) +
                          c1 * (yz[q] - yz[q - sy]) + c2 * (yz[q + sy] - yz[q - 2 * sy]) +
                          c1 * (zz[q + sz] - zz[q]) + c2 * (zz[q + 2 * sz] - zz[q - sz]));
 }
@@ -79,59 +87,57 @@ __device__ inline double newton(double vtilde, double cdelta, double uw, double 
 //    fd3d_deriv.f90:284-508), mirror v(nyt) = v(nyt-1), then tasu1/tasw1
 //    at j = nyt (:1109-1234);
 //  * other columns: the v mirror only, reading v(nyt-1) before pml_vel.
-// Also advances the step counter read by the later kernels of this step.
 // ---------------------------------------------------------------------------
 // FVW = false compiles the slip-weakening path only (for callers that
 // know no problem uses rate-and-state friction); results are unchanged.
 template <int BY, bool FVW = true>
-__device__ inline void vel_fault_body(const Dims &g, const Coef &c, const Fields &f, const Fault &fl, const FvwScalars &fv, Ctl *ctl, int gbx, int gby, int lx, int ly) {
+__device__ inline void vel_fault_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, const Fault &fl, const FvwScalars &fv, Ctl *ctl, int gbx, int gby, int lx, int ly) {
   const int i = 1 + gbx * BX + lx;
   const int k = 1 + gby * BY + ly;
-  if (gbx == 0 && gby == 0 && lx == 0 && ly == 0) ctl->it += 1;
   if (i > g.nxt || k > g.nzt) return;
   const int sy = g.sy, sz = g.sz;
   const int qm = I3(g, i, g.nyt - 1, k);
   const bool fault = i >= g.nabc + 1 && i <= g.nxt - g.nabc && k >= g.nabc + 1 && k <= g.nzt - g.nfs;
   if (!fault) {
-    f.v[qm + sy] = f.v[qm];
+    fn.v[qm + sy] = fo.v[qm];
     return;
   }
   const int p = I2(g, i, k);
   // bnd2d at j = nyt-1
   {
     const int q = qm;
-    const float *xx = f.xx, *yy = f.yy, *zz = f.zz, *xy = f.xy, *yz = f.yz, *xz = f.xz;
-    const float b = c.dth / f.d[q];
+    const float *xx = fo.xx, *yy = fo.yy, *zz = fo.zz, *xy = fo.xy, *yz = fo.yz, *xz = fo.xz;
+    const float b = c.dth / fo.d[q];
     float pdx = ((xx[q] - xx[q - 1]) + (xy[q] - xy[q - sy])) + (xz[q] - xz[q - sz]);
     float a = pdx - fl.au1[p];
-    f.u[q] = f.u[q] + b * (pdx + c.damp_s * a);
+    fn.u[q] = fo.u[q] + b * (pdx + c.damp_s * a);
     fl.au1[p] = pdx;
     float pdy = ((xy[q + 1] - xy[q]) + (yy[q + sy] - yy[q])) + (yz[q] - yz[q - sz]);
     a = pdy - fl.av1[p];
-    const float vnew = f.v[q] + b * (pdy + c.damp_s * a);
-    f.v[q] = vnew;
+    const float vnew = fo.v[q] + b * (pdy + c.damp_s * a);
+    fn.v[q] = vnew;
     fl.av1[p] = pdy;
     float pdz = ((xz[q + 1] - xz[q]) + (yz[q] - yz[q - sy])) + (zz[q + sz] - zz[q]);
     a = pdz - fl.aw1[p];
-    f.w[q] = f.w[q] + b * (pdz + c.damp_s * a);
+    fn.w[q] = fo.w[q] + b * (pdz + c.damp_s * a);
     fl.aw1[p] = pdz;
-    f.v[q + sy] = vnew;
+    fn.v[q + sy] = vnew;
   }
   // tasu1 / tasw1 at j = nyt
   const int q = qm + sy;
   if (!FVW || !fv.fvw) {
-    const float b = c.dth / f.d[q];
-    f.u[q] = f.u[q] + (b * 2.0f) * (fl.tx[p] + fl.RFx[p]);
-    f.w[q] = f.w[q] + b * (2.0f * (fl.tz[p] + fl.RFz[p]));
+    const float b = c.dth / fo.d[q];
+    fn.u[q] = fo.u[q] + (b * 2.0f) * (fl.tx[p] + fl.RFx[p]);
+    fn.w[q] = fo.w[q] + b * (2.0f * (fl.tz[p] + fl.RFz[p]));
     return;
   }
   const double dthF = (double)c.dt / (double)c.dh;
-  const double dd = (double)f.d[q];
+  const double dd = (double)fo.d[q];
   const double two_dth_d = 2.0 * (dthF / dd);
   // tasu1: X-node (fd3d_deriv.f90:1141-1162)
   {
     const int pim = p - 1, pkm = p - g.nxt, pikm = p - 1 - g.nxt;
-    const double uc = (double)f.u[q], wx = (double)fl.wX[p];
+    const double uc = (double)fo.u[q], wx = (double)fl.wX[p];
     const double aX = fl.aX[p], psiX = fl.psiX[p];
     const double sr = sqrt(sqd(2.0 * (fabs(wx) + fabs(fv.wini))) + sqd(2.0 * (fabs(uc) + fabs(fv.uini))));
     const double cdelta = (two_dth_d * fv.Sn) * aX;
@@ -143,13 +149,13 @@ __device__ inline void vel_fault_body(const Dims &g, const Coef &c, const Fields
     const double ex = exp(-psiX / aX);
     uw = newton(vtilde, cdelta, uw, fv.v0, ex, fv.newton_max, ctl);
     // ((tx+T0X)*(-sinh))*v0*ex/tabsX + uini, association as the reference
-    f.u[q] = (float)((((((double)fl.tx[p] + (double)fl.T0X[p]) * (-sinh(uw))) * fv.v0) * ex) /
+    fn.u[q] = (float)((((((double)fl.tx[p] + (double)fl.T0X[p]) * (-sinh(uw))) * fv.v0) * ex) /
                          (double)fl.tabsX[p] + fv.uini);
   }
   // tasw1: Z-node (fd3d_deriv.f90:1204-1226)
   {
     const int pip = p + 1, pkp = p + g.nxt, pikp = p + 1 + g.nxt;
-    const double wc = (double)f.w[q], uz = (double)fl.uZ[p];
+    const double wc = (double)fo.w[q], uz = (double)fl.uZ[p];
     const double aZ = fl.aZ[p], psiZ = fl.psiZ[p];
     const double sr = sqrt(sqd(2.0 * (wc - fv.wini)) + sqd(2.0 * (uz - fv.uini)));
     const double cdelta = (two_dth_d * fv.Sn) * aZ;
@@ -162,7 +168,7 @@ __device__ inline void vel_fault_body(const Dims &g, const Coef &c, const Fields
                                 sqd(uz + two_dth_d * (rfx4 - t0x4)));
     const double ex = exp(-psiZ / aZ);
     uw = newton(vtilde, cdelta, uw, fv.v0, ex, fv.newton_max, ctl);
-    f.w[q] = (float)((((double)fl.tz[p] + (double)fl.T0Z[p]) * (-((sinh(uw) * fv.v0) * ex))) /
+    fn.w[q] = (float)((((double)fl.tz[p] + (double)fl.T0Z[p]) * (-((sinh(uw) * fv.v0) * ex))) /
                          (double)fl.tabsZ[p] + fv.wini);
   }
 }
@@ -181,7 +187,7 @@ __device__ inline int pml_region(const Pml &pm, int blk) {
 // fd3d_deriv.f90:1927-2299): p <- [p(1-w) + ((dt/rho)*dsigma)/dh]/(1+w),
 // field = sum of splits.  u and w to j = je_uw, v to je_v.
 // ---------------------------------------------------------------------------
-__device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &f, const Pml &pm, int gbx, int lx) {
+__device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, const Pml &pm, int gbx, int lx) {
   const int r = pml_region(pm, gbx);
   const PmlRegion &R = pm.r[r];
   const int local = (gbx - R.blk0) * kPmlBlock + lx;
@@ -200,38 +206,51 @@ __device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &
   float *s = pm.splits + R.split_off + local;
   const long long nc = R.ncell;
   const float dt = c.dt, dh = c.dh;
-  const float dtd = dt / f.d[q];
+  const bool hasv = j <= R.je_v;
+  // every load before the first store (the stores could alias the inputs
+  // as far as the compiler knows, which would serialise the loads)
+  const float d0 = fo.d[q];
+  const float su1 = s[0 * nc], su2 = s[1 * nc], su3 = s[2 * nc];
+  const float sw1 = s[6 * nc], sw2 = s[7 * nc], sw3 = s[8 * nc];
+  const float xx0 = fo.xx[q], xxm = fo.xx[q - 1], xy0 = fo.xy[q], xym = fo.xy[q - sy];
+  const float xz0 = fo.xz[q], xzmz = fo.xz[q - sz], xzp = fo.xz[q + 1];
+  const float yz0 = fo.yz[q], yzmy = fo.yz[q - sy], zz0 = fo.zz[q], zzp = fo.zz[q + sz];
+  const float pamx = amx[i2], papx = apx[i2], pamxS = amxS[i2], papxS = apxS[i2];
+  const float pamy = amy[j2], papy = apy[j2], pamz = amz[k2], papz = apz[k2];
+  float sv1 = 0, sv2 = 0, sv3 = 0, xyp = 0, yy0 = 0, yyp = 0, yzmz = 0;
+  if (hasv) {
+    sv1 = s[3 * nc]; sv2 = s[4 * nc]; sv3 = s[5 * nc];
+    xyp = fo.xy[q + 1]; yy0 = fo.yy[q]; yyp = fo.yy[q + sy]; yzmz = fo.yz[q - sz];
+  }
+  const float dtd = dt / d0;
   float pt, s1, s2, s3;
   // u (x, y, z profiles)
-  s1 = s[0 * nc]; s2 = s[1 * nc]; s3 = s[2 * nc];
-  pt = s1 * amx[i2] + (dtd * (f.xx[q] - f.xx[q - 1])) / dh;
-  s1 = pt / apx[i2];
-  pt = s2 * amy[j2] + (dtd * (f.xy[q] - f.xy[q - sy])) / dh;
-  s2 = pt / apy[j2];
-  pt = s3 * amz[k2] + (dtd * (f.xz[q] - f.xz[q - sz])) / dh;
-  s3 = pt / apz[k2];
+  pt = su1 * pamx + (dtd * (xx0 - xxm)) / dh;
+  s1 = pt / papx;
+  pt = su2 * pamy + (dtd * (xy0 - xym)) / dh;
+  s2 = pt / papy;
+  pt = su3 * pamz + (dtd * (xz0 - xzmz)) / dh;
+  s3 = pt / papz;
   s[0 * nc] = s1; s[1 * nc] = s2; s[2 * nc] = s3;
-  f.u[q] = (s1 + s2) + s3;
+  fn.u[q] = (s1 + s2) + s3;
   // w (xS, y, z profiles)
-  s1 = s[6 * nc]; s2 = s[7 * nc]; s3 = s[8 * nc];
-  pt = s1 * amxS[i2] + (dtd * (f.xz[q + 1] - f.xz[q])) / dh;
-  s1 = pt / apxS[i2];
-  pt = s2 * amy[j2] + (dtd * (f.yz[q] - f.yz[q - sy])) / dh;
-  s2 = pt / apy[j2];
-  pt = s3 * amz[k2] + (dtd * (f.zz[q + sz] - f.zz[q])) / dh;
-  s3 = pt / apz[k2];
+  pt = sw1 * pamxS + (dtd * (xzp - xz0)) / dh;
+  s1 = pt / papxS;
+  pt = sw2 * pamy + (dtd * (yz0 - yzmy)) / dh;
+  s2 = pt / papy;
+  pt = sw3 * pamz + (dtd * (zzp - zz0)) / dh;
+  s3 = pt / papz;
   s[6 * nc] = s1; s[7 * nc] = s2; s[8 * nc] = s3;
-  f.w[q] = (s1 + s2) + s3;
-  if (j <= R.je_v) {
-    s1 = s[3 * nc]; s2 = s[4 * nc]; s3 = s[5 * nc];
-    pt = s1 * amxS[i2] + (dtd * (f.xy[q + 1] - f.xy[q])) / dh;
-    s1 = pt / apxS[i2];
-    pt = s2 * amy[j2] + (dtd * (f.yy[q + sy] - f.yy[q])) / dh;
-    s2 = pt / apy[j2];
-    pt = s3 * amz[k2] + (dtd * (f.yz[q] - f.yz[q - sz])) / dh;
-    s3 = pt / apz[k2];
+  fn.w[q] = (s1 + s2) + s3;
+  if (hasv) {
+    pt = sv1 * pamxS + (dtd * (xyp - xy0)) / dh;
+    s1 = pt / papxS;
+    pt = sv2 * pamy + (dtd * (yyp - yy0)) / dh;
+    s2 = pt / papy;
+    pt = sv3 * pamz + (dtd * (yz0 - yzmz)) / dh;
+    s3 = pt / papz;
     s[3 * nc] = s1; s[4 * nc] = s2; s[5 * nc] = s3;
-    f.v[q] = (s1 + s2) + s3;
+    fn.v[q] = (s1 + s2) + s3;
   }
 }
 
@@ -240,29 +259,29 @@ __device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &
 // launches because w at the ghost row reads the new ghost u and v.
 // ---------------------------------------------------------------------------
 template <int BY>
-__device__ inline void fuvw_uv_body(const Dims &g, const Fields &f, int gbx, int gby, int lx, int ly) {
+__device__ inline void fuvw_uv_body(const Dims &g, const Fields &fo, const Fields &fn, int gbx, int gby, int lx, int ly) {
   const int i = 1 + gbx * BX + lx;
   const int j = 1 + gby * BY + ly;
   if (i < 2 || i > g.nxt - 1 || j > g.nyt) return;
   const int ks = g.nzt - g.nfs;
   const int q = I3(g, i, j, ks), sz = g.sz, sy = g.sy;
-  f.u[q + sz] = f.u[q] - (f.w[q] - f.w[q - 1]);
-  if (j >= 2 && j <= g.nyt - 1) f.v[q + sz] = f.v[q] - (f.w[q + sy] - f.w[q]);
+  fn.u[q + sz] = fn.u[q] - (fn.w[q] - fn.w[q - 1]);
+  if (j >= 2 && j <= g.nyt - 1) fn.v[q + sz] = fn.v[q] - (fn.w[q + sy] - fn.w[q]);
 }
 
 template <int BY>
-__device__ inline void fuvw_w_body(const Dims &g, const Fields &f, int gbx, int gby, int lx, int ly) {
+__device__ inline void fuvw_w_body(const Dims &g, const Fields &fo, const Fields &fn, int gbx, int gby, int lx, int ly) {
   const int i = 1 + gbx * BX + lx;
   const int j = 1 + gby * BY + ly;
   if (i < 2 || i > g.nxt - 2 || j < 2 || j > g.nyt) return;
   const int ks = g.nzt - g.nfs;
   const int q = I3(g, i, j, ks), sz = g.sz, sy = g.sy;
-  const float xl = f.lam[q + sz], xm = f.mu[q + sz];
+  const float xl = fn.lam[q + sz], xm = fn.mu[q + sz];
   const float a = 2.0f * xl, b = xl + 2.0f * xm;
-  const float *u = f.u, *v = f.v;
+  const float *u = fn.u, *v = fn.v;
   const float s = ((((((u[q + 1] - u[q]) + u[q + 1 + sz]) - u[q + sz]) + v[q]) - v[q - sy]) +
                    v[q + sz]) - v[q - sy + sz];
-  f.w[q + sz] = f.w[q - sz] - (a / b) * s;
+  fn.w[q + sz] = fn.w[q - sz] - (a / b) * s;
 }
 
 // ---------------------------------------------------------------------------
@@ -270,30 +289,30 @@ __device__ inline void fuvw_w_body(const Dims &g, const Fields &f, int gbx, int 
 // interior velocities.  Bytes per cell: reads u, v, w, lam, mu + 6 stresses;
 // writes 6 stresses.
 // ---------------------------------------------------------------------------
-__device__ inline void str_interior_body(const Dims &g, const Coef &c, const Fields &f, int gbx, int gby, int gbz, int lx, int ly) {
+__device__ inline void str_interior_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, int gbx, int gby, int gbz, int lx, int ly) {
   const int i = g.nabc + 1 + gbx * BX + lx;
   const int j = g.nabc + 1 + gby * BYF + ly;
   const int k = g.nabc + 1 + gbz;
   if (i > g.nxt - g.nabc || j > g.nyt - 2) return;
   const int q = I3(g, i, j, k), sy = g.sy, sz = g.sz;
   const float c1 = c.c1, c2 = c.c2, dth = c.dth;
-  const float *u = f.u, *v = f.v, *w = f.w, *mu = f.mu;
-  const float xl = f.lam[q], xm = mu[q];
+  const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fo.mu;
+  const float xl = fo.lam[q], xm = mu[q];
   const float a = xl + 2.0f * xm, b = xl;
   const float dux1 = c1 * (u[q + 1] - u[q]), dux2 = c2 * (u[q + 2] - u[q - 1]);
   const float dvy1 = c1 * (v[q] - v[q - sy]), dvy2 = c2 * (v[q + sy] - v[q - 2 * sy]);
   const float dwz1 = c1 * (w[q] - w[q - sz]), dwz2 = c2 * (w[q + sz] - w[q - 2 * sz]);
-  f.xx[q] = (f.xx[q] + (dth * a) * (dux1 + dux2)) + (dth * b) * (((dvy1 + dvy2) + dwz1) + dwz2);
-  f.yy[q] = (f.yy[q] + (dth * a) * (dvy1 + dvy2)) + (dth * b) * (((dux1 + dux2) + dwz1) + dwz2);
-  f.zz[q] = (f.zz[q] + (dth * a) * (dwz1 + dwz2)) + (dth * b) * (((dux1 + dux2) + dvy1) + dvy2);
+  fn.xx[q] = (fo.xx[q] + (dth * a) * (dux1 + dux2)) + (dth * b) * (((dvy1 + dvy2) + dwz1) + dwz2);
+  fn.yy[q] = (fo.yy[q] + (dth * a) * (dvy1 + dvy2)) + (dth * b) * (((dux1 + dux2) + dwz1) + dwz2);
+  fn.zz[q] = (fo.zz[q] + (dth * a) * (dwz1 + dwz2)) + (dth * b) * (((dux1 + dux2) + dvy1) + dvy2);
   float xmu = 0.5f * (xm + mu[q + sy]);
-  f.xy[q] = f.xy[q] + (dth * xmu) * (((c1 * (u[q + sy] - u[q]) + c2 * (u[q + 2 * sy] - u[q - sy])) +
+  fn.xy[q] = fo.xy[q] + (dth * xmu) * (((c1 * (u[q + sy] - u[q]) + c2 * (u[q + 2 * sy] - u[q - sy])) +
                                       c1 * (v[q] - v[q - 1])) + c2 * (v[q + 1] - v[q - 2]));
   xmu = 0.5f * (xm + mu[q + sz]);
-  f.xz[q] = f.xz[q] + (dth * xmu) * (((c1 * (u[q + sz] - u[q]) + c2 * (u[q + 2 * sz] - u[q - sz])) +
+  fn.xz[q] = fo.xz[q] + (dth * xmu) * (((c1 * (u[q + sz] - u[q]) + c2 * (u[q + 2 * sz] - u[q - sz])) +
                                       c1 * (w[q] - w[q - 1])) + c2 * (w[q + 1] - w[q - 2]));
   xmu = 0.5f * (xm + mu[q + 1 + sy + sz]);
-  f.yz[q] = f.yz[q] + (dth * xmu) * (((c1 * (v[q + sz] - v[q]) + c2 * (v[q + 2 * sz] - v[q - sz])) +
+  fn.yz[q] = fo.yz[q] + (dth * xmu) * (((c1 * (v[q + sz] - v[q]) + c2 * (v[q + 2 * sz] - v[q - sz])) +
                                       c1 * (w[q + sy] - w[q])) + c2 * (w[q + 2 * sy] - w[q - sy]));
 }
 
@@ -305,7 +324,7 @@ __device__ inline void str_interior_body(const Dims &g, const Coef &c, const Fie
 //  * other columns: the xy/yz mirror only, before pml_str.
 // ---------------------------------------------------------------------------
 template <int BY>
-__device__ inline void str_fault_body(const Dims &g, const Coef &c, const Fields &f, const Fault &fl, int gbx, int gby, int lx, int ly) {
+__device__ inline void str_fault_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, const Fault &fl, int gbx, int gby, int lx, int ly) {
   const int i = 1 + gbx * BX + lx;
   const int k = 1 + gby * BY + ly;
   if (i > g.nxt || k > g.nzt) return;
@@ -313,39 +332,39 @@ __device__ inline void str_fault_body(const Dims &g, const Coef &c, const Fields
   const int q = I3(g, i, g.nyt - 1, k);
   const bool fault = i >= g.nabc + 1 && i <= g.nxt - g.nabc && k >= g.nabc + 1 && k <= g.nzt - g.nfs;
   if (!fault) {
-    f.xy[q + sy] = f.xy[q];
-    f.yz[q + sy] = f.yz[q];
+    fn.xy[q + sy] = fo.xy[q];
+    fn.yz[q + sy] = fo.yz[q];
     return;
   }
-  const float *u = f.u, *v = f.v, *w = f.w, *mu = f.mu;
+  const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fo.mu;
   {
     const float dth = c.dth;
-    const float xl = f.lam[q], xm = mu[q];
+    const float xl = fo.lam[q], xm = mu[q];
     const float a = xl + 2.0f * xm, b = xl;
-    f.xx[q] = (f.xx[q] + (dth * a) * (u[q + 1] - u[q])) + (dth * b) * (((v[q] - v[q - sy]) + w[q]) - w[q - sz]);
-    f.yy[q] = (f.yy[q] + (dth * a) * (v[q] - v[q - sy])) + (dth * b) * (((u[q + 1] - u[q]) + w[q]) - w[q - sz]);
-    f.zz[q] = (f.zz[q] + (dth * a) * (w[q] - w[q - sz])) + (dth * b) * (((u[q + 1] - u[q]) + v[q]) - v[q - sy]);
+    fn.xx[q] = (fo.xx[q] + (dth * a) * (u[q + 1] - u[q])) + (dth * b) * (((v[q] - v[q - sy]) + w[q]) - w[q - sz]);
+    fn.yy[q] = (fo.yy[q] + (dth * a) * (v[q] - v[q - sy])) + (dth * b) * (((u[q + 1] - u[q]) + w[q]) - w[q - sz]);
+    fn.zz[q] = (fo.zz[q] + (dth * a) * (w[q] - w[q - sz])) + (dth * b) * (((u[q + 1] - u[q]) + v[q]) - v[q - sy]);
     const float dthh = c.dthh;
     float xmu = xm + mu[q + sy];
-    const float xyn = f.xy[q] + (dthh * xmu) * (((u[q + sy] - u[q]) + v[q]) - v[q - 1]);
-    f.xy[q] = xyn;
+    const float xyn = fo.xy[q] + (dthh * xmu) * (((u[q + sy] - u[q]) + v[q]) - v[q - 1]);
+    fn.xy[q] = xyn;
     xmu = xm + mu[q + sz];
-    f.xz[q] = f.xz[q] + (dthh * xmu) * (((u[q + sz] - u[q]) + w[q]) - w[q - 1]);
+    fn.xz[q] = fo.xz[q] + (dthh * xmu) * (((u[q + sz] - u[q]) + w[q]) - w[q - 1]);
     xmu = xm + mu[q + 1 + sy + sz];
-    const float yzn = f.yz[q] + (dthh * xmu) * (((v[q + sz] - v[q]) + w[q + sy]) - w[q]);
-    f.yz[q] = yzn;
-    f.xy[q + sy] = xyn;
-    f.yz[q + sy] = yzn;
+    const float yzn = fo.yz[q] + (dthh * xmu) * (((v[q + sz] - v[q]) + w[q + sy]) - w[q]);
+    fn.yz[q] = yzn;
+    fn.xy[q + sy] = xyn;
+    fn.yz[q + sy] = yzn;
   }
   {
     const int qf = q + sy;
     const float c1 = c.c1, c2 = c.c2;
     float xmu = mu[qf] + mu[qf + sz];
-    f.xz[qf] = f.xz[qf] + (c.dthh * xmu) *
+    fn.xz[qf] = fo.xz[qf] + (c.dthh * xmu) *
                               (((c1 * (u[qf + sz] - u[qf]) + c2 * (u[qf + 2 * sz] - u[qf - sz])) +
                                 c1 * (w[qf] - w[qf - 1])) + c2 * (w[qf + 1] - w[qf - 2]));
     const float dth = c.dth;
-    const float xl = f.lam[qf], xm = mu[qf];
+    const float xl = fo.lam[qf], xm = mu[qf];
     const float a = xl + 2.0f * xm, b = xl;
     const float diff1 = c1 * (u[qf + 1] - u[qf]) + c2 * (u[qf + 2] - u[qf - 1]);
     const float diff3 = c1 * (w[qf] - w[qf - sz]) + c2 * (w[qf + sz] - w[qf - 2 * sz]);
@@ -354,9 +373,9 @@ __device__ inline void str_fault_body(const Dims &g, const Coef &c, const Fields
     const float v1t = vm - (b * (diff1 + diff3)) / (2.0f * a);
     fl.v1t[p] = v1t;
     const float dv = v1t - vm;
-    f.xx[qf] = (f.xx[qf] + (dth * a) * diff1) + (dth * b) * (2.0f * dv + diff3);
-    f.yy[qf] = (f.yy[qf] + ((dth * a) * 2.0f) * dv) + (dth * b) * (diff1 + diff3);
-    f.zz[qf] = (f.zz[qf] + (dth * a) * diff3) + (dth * b) * (diff1 + 2.0f * dv);
+    fn.xx[qf] = (fo.xx[qf] + (dth * a) * diff1) + (dth * b) * (2.0f * dv + diff3);
+    fn.yy[qf] = (fo.yy[qf] + ((dth * a) * 2.0f) * dv) + (dth * b) * (diff1 + diff3);
+    fn.zz[qf] = (fo.zz[qf] + (dth * a) * diff3) + (dth * b) * (diff1 + 2.0f * dv);
   }
 }
 
@@ -365,7 +384,7 @@ __device__ inline void str_fault_body(const Dims &g, const Coef &c, const Fields
 // fd3d_deriv.f90:2304-3145), with the top-layer xz/yz split zeroing of
 // regions 1 (always) and 2, 3 (free surface only) fused after the update.
 // ---------------------------------------------------------------------------
-__device__ inline void pml_str_body(const Dims &g, const Coef &c, const Fields &f, const Pml &pm, int gbx, int lx) {
+__device__ inline void pml_str_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, const Pml &pm, int gbx, int lx) {
   const int r = pml_region(pm, gbx);
   const PmlRegion &R = pm.r[r];
   const int local = (gbx - R.blk0) * kPmlBlock + lx;
@@ -384,54 +403,65 @@ __device__ inline void pml_str_body(const Dims &g, const Coef &c, const Fields &
   float *s = pm.splits + R.split_off + local;
   const long long nc = R.ncell;
   const float dt = c.dt, dh = c.dh;
-  const float *u = f.u, *v = f.v, *w = f.w, *mu = f.mu;
-  const float xl = f.lam[q], xm = mu[q];
-  const float a = xl + 2.0f * xm, b = xl;
-  const float diff1 = u[q + 1] - u[q], diff2 = v[q] - v[q - sy], diff3 = w[q] - w[q - sz];
+  const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fn.mu;
+  const bool hxy = j <= R.je_v, hxz = j <= R.je_xz;
+  // every load before the first store (see pml_vel_body)
+  const float xl = fn.lam[q], xm = mu[q];
+  const float u0 = u[q], up = u[q + 1], v0 = v[q], vmy = v[q - sy], w0 = w[q], wmz = w[q - sz];
   const float ax = amxS[i2], bx = apxS[i2], ay = amy[j2], by = apy[j2], az = amz[k2], bz = apz[k2];
-  float pt, s1, s2, s3;
-  s1 = s[9 * nc]; s2 = s[10 * nc]; s3 = s[11 * nc];
-  pt = s1 * ax + ((dt * a) * diff1) / dh; s1 = pt / bx;
-  pt = s2 * ay + ((dt * b) * diff2) / dh; s2 = pt / by;
-  pt = s3 * az + ((dt * b) * diff3) / dh; s3 = pt / bz;
-  s[9 * nc] = s1; s[10 * nc] = s2; s[11 * nc] = s3;
-  f.xx[q] = (s1 + s2) + s3;
-  s1 = s[12 * nc]; s2 = s[13 * nc]; s3 = s[14 * nc];
-  pt = s1 * ax + ((dt * b) * diff1) / dh; s1 = pt / bx;
-  pt = s2 * ay + ((dt * a) * diff2) / dh; s2 = pt / by;
-  pt = s3 * az + ((dt * b) * diff3) / dh; s3 = pt / bz;
-  s[12 * nc] = s1; s[13 * nc] = s2; s[14 * nc] = s3;
-  f.yy[q] = (s1 + s2) + s3;
-  s1 = s[15 * nc]; s2 = s[16 * nc]; s3 = s[17 * nc];
-  pt = s1 * ax + ((dt * b) * diff1) / dh; s1 = pt / bx;
-  pt = s2 * ay + ((dt * b) * diff2) / dh; s2 = pt / by;
-  pt = s3 * az + ((dt * a) * diff3) / dh; s3 = pt / bz;
-  s[15 * nc] = s1; s[16 * nc] = s2; s[17 * nc] = s3;
-  f.zz[q] = (s1 + s2) + s3;
-  const bool zero_top = R.ztop && k2 == R.nk;
-  if (j <= R.je_v) {
-    const float xmu = (xm + mu[q + sy]) / 2.0f;
-    s1 = s[18 * nc]; s2 = s[19 * nc];
-    pt = s1 * amyS[j2] + ((dt * xmu) * (u[q + sy] - u[q])) / dh; s1 = pt / apyS[j2];
-    pt = s2 * amx[i2] + ((dt * xmu) * (v[q] - v[q - 1])) / dh; s2 = pt / apx[i2];
-    s[18 * nc] = s1; s[19 * nc] = s2;
-    f.xy[q] = s1 + s2;
+  const float sxx1 = s[9 * nc], sxx2 = s[10 * nc], sxx3 = s[11 * nc];
+  const float syy1 = s[12 * nc], syy2 = s[13 * nc], syy3 = s[14 * nc];
+  const float szz1 = s[15 * nc], szz2 = s[16 * nc], szz3 = s[17 * nc];
+  float mxy = 0, uy = 0, vmx = 0, sxy1 = 0, sxy2 = 0, pamyS = 0, papyS = 0, pamx = 0, papx = 0;
+  float mxz = 0, uz = 0, wmx = 0, sxz1 = 0, sxz2 = 0, pamzS = 0, papzS = 0;
+  float myz = 0, vz = 0, wy = 0, syz1 = 0, syz2 = 0;
+  if (hxy || hxz) { pamx = amx[i2]; papx = apx[i2]; }
+  if (hxy) {
+    mxy = mu[q + sy]; uy = u[q + sy]; vmx = v[q - 1]; sxy1 = s[18 * nc]; sxy2 = s[19 * nc];
+    pamyS = amyS[j2]; papyS = apyS[j2];
+    myz = mu[q + 1 + sy + sz]; vz = v[q + sz]; wy = w[q + sy]; syz1 = s[22 * nc]; syz2 = s[23 * nc];
   }
-  if (j <= R.je_xz) {
-    const float xmu = (xm + mu[q + sz]) / 2.0f;
-    s1 = s[20 * nc]; s2 = s[21 * nc];
-    pt = s1 * amzS[k2] + ((dt * xmu) * (u[q + sz] - u[q])) / dh; s1 = pt / apzS[k2];
-    pt = s2 * amx[i2] + ((dt * xmu) * (w[q] - w[q - 1])) / dh; s2 = pt / apx[i2];
-    f.xz[q] = s1 + s2;
+  if (hxz) { mxz = mu[q + sz]; uz = u[q + sz]; wmx = w[q - 1]; sxz1 = s[20 * nc]; sxz2 = s[21 * nc]; }
+  if (hxy || hxz) { pamzS = amzS[k2]; papzS = apzS[k2]; }
+  const float a = xl + 2.0f * xm, b = xl;
+  const float diff1 = up - u0, diff2 = v0 - vmy, diff3 = w0 - wmz;
+  float pt, s1, s2, s3;
+  pt = sxx1 * ax + ((dt * a) * diff1) / dh; s1 = pt / bx;
+  pt = sxx2 * ay + ((dt * b) * diff2) / dh; s2 = pt / by;
+  pt = sxx3 * az + ((dt * b) * diff3) / dh; s3 = pt / bz;
+  s[9 * nc] = s1; s[10 * nc] = s2; s[11 * nc] = s3;
+  fn.xx[q] = (s1 + s2) + s3;
+  pt = syy1 * ax + ((dt * b) * diff1) / dh; s1 = pt / bx;
+  pt = syy2 * ay + ((dt * a) * diff2) / dh; s2 = pt / by;
+  pt = syy3 * az + ((dt * b) * diff3) / dh; s3 = pt / bz;
+  s[12 * nc] = s1; s[13 * nc] = s2; s[14 * nc] = s3;
+  fn.yy[q] = (s1 + s2) + s3;
+  pt = szz1 * ax + ((dt * b) * diff1) / dh; s1 = pt / bx;
+  pt = szz2 * ay + ((dt * b) * diff2) / dh; s2 = pt / by;
+  pt = szz3 * az + ((dt * a) * diff3) / dh; s3 = pt / bz;
+  s[15 * nc] = s1; s[16 * nc] = s2; s[17 * nc] = s3;
+  fn.zz[q] = (s1 + s2) + s3;
+  const bool zero_top = R.ztop && k2 == R.nk;
+  if (hxy) {
+    const float xmu = (xm + mxy) / 2.0f;
+    pt = sxy1 * pamyS + ((dt * xmu) * (uy - u0)) / dh; s1 = pt / papyS;
+    pt = sxy2 * pamx + ((dt * xmu) * (v0 - vmx)) / dh; s2 = pt / papx;
+    s[18 * nc] = s1; s[19 * nc] = s2;
+    fn.xy[q] = s1 + s2;
+  }
+  if (hxz) {
+    const float xmu = (xm + mxz) / 2.0f;
+    pt = sxz1 * pamzS + ((dt * xmu) * (uz - u0)) / dh; s1 = pt / papzS;
+    pt = sxz2 * pamx + ((dt * xmu) * (w0 - wmx)) / dh; s2 = pt / papx;
+    fn.xz[q] = s1 + s2;
     if (zero_top && j2 <= R.nzx) { s1 = 0.0f; s2 = 0.0f; }
     s[20 * nc] = s1; s[21 * nc] = s2;
   }
-  if (j <= R.je_v) {
-    const float xmu = (xm + mu[q + 1 + sy + sz]) / 2.0f;
-    s1 = s[22 * nc]; s2 = s[23 * nc];
-    pt = s1 * amzS[k2] + ((dt * xmu) * (v[q + sz] - v[q])) / dh; s1 = pt / apzS[k2];
-    pt = s2 * amyS[j2] + ((dt * xmu) * (w[q + sy] - w[q])) / dh; s2 = pt / apyS[j2];
-    f.yz[q] = s1 + s2;
+  if (hxy) {
+    const float xmu = (xm + myz) / 2.0f;
+    pt = syz1 * pamzS + ((dt * xmu) * (vz - v0)) / dh; s1 = pt / papzS;
+    pt = syz2 * pamyS + ((dt * xmu) * (wy - w0)) / dh; s2 = pt / papyS;
+    fn.yz[q] = s1 + s2;
     if (zero_top && j2 <= R.nzy) { s1 = 0.0f; s2 = 0.0f; }
     s[22 * nc] = s1; s[23 * nc] = s2;
   }
@@ -439,34 +469,34 @@ __device__ inline void pml_str_body(const Dims &g, const Coef &c, const Fields &
 
 // Free-surface stress imaging (fres, fd3d_deriv.f90:1067-1104).
 template <int BY>
-__device__ inline void fres_body(const Dims &g, const Fields &f, int gbx, int gby, int lx, int ly) {
+__device__ inline void fres_body(const Dims &g, const Fields &fo, const Fields &fn, int gbx, int gby, int lx, int ly) {
   const int i = 1 + gbx * BX + lx;
   const int j = 1 + gby * BY + ly;
   if (i > g.nxt || j > g.nyt) return;
   const int ks = g.nzt - g.nfs;
   const int q = I3(g, i, j, ks), sz = g.sz;
-  f.zz[q + sz] = -f.zz[q];
-  f.zz[q + 2 * sz] = -f.zz[q - sz];
-  f.xz[q + sz] = -f.xz[q - sz];
-  f.xz[q + 2 * sz] = -f.xz[q - 2 * sz];
-  f.yz[q + sz] = -f.yz[q - sz];
-  f.yz[q + 2 * sz] = -f.yz[q - 2 * sz];
-  f.xz[q] = 0.0f;
-  f.yz[q] = 0.0f;
+  fn.zz[q + sz] = -fn.zz[q];
+  fn.zz[q + 2 * sz] = -fn.zz[q - sz];
+  fn.xz[q + sz] = -fn.xz[q - sz];
+  fn.xz[q + 2 * sz] = -fn.xz[q - 2 * sz];
+  fn.yz[q + sz] = -fn.yz[q - sz];
+  fn.yz[q + 2 * sz] = -fn.yz[q - 2 * sz];
+  fn.xz[q] = 0.0f;
+  fn.yz[q] = 0.0f;
 }
 
 // ---------------------------------------------------------------------------
 // Trial traction (fd3d_theo.f90:232-308) on fault nodes.
 // ---------------------------------------------------------------------------
 template <int BY>
-__device__ inline void traction_body(const Dims &g, const Coef &c, const Fields &f, const Fault &fl, int gbx, int gby, int lx, int ly) {
+__device__ inline void traction_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, const Fault &fl, int gbx, int gby, int lx, int ly) {
   const int i = g.nabc + 1 + gbx * BX + lx;
   const int k = g.nabc + 1 + gby * BY + ly;
   const int ks = g.nzt - g.nfs;
   if (i > g.nxt - g.nabc || k > ks) return;
   const int q = I3(g, i, g.nyt, k), sy = g.sy, sz = g.sz, p = I2(g, i, k);
   const float c1 = c.c1, c2 = c.c2, dht = c.dht;
-  const float *xx = f.xx, *zz = f.zz, *xy = f.xy, *yz = f.yz, *xz = f.xz;
+  const float *xx = fn.xx, *zz = fn.zz, *xy = fn.xy, *yz = fn.yz, *xz = fn.xz;
   float pdz;
   const bool top_free = !g.fspace && k == ks;
   if (!top_free)
@@ -477,18 +507,18 @@ __device__ inline void traction_body(const Dims &g, const Coef &c, const Fields 
   float av = c.damp_s * (pdz - fl.avdz[p]);
   float rf = pdz + av;
   fl.RFz[p] = rf;
-  const float dd = f.d[q];
+  const float dd = fn.d[q];
   if (!top_free)
-    fl.tz[p] = -rf - ((0.5f * dd) * dht) * f.w[q];
+    fl.tz[p] = -rf - ((0.5f * dd) * dht) * fn.w[q];
   else
-    fl.tz[p] = -rf - ((((0.5f * dd) * dht) * f.w[q]) / 2.0f);
+    fl.tz[p] = -rf - ((((0.5f * dd) * dht) * fn.w[q]) / 2.0f);
   fl.avdz[p] = pdz;
   const float pdx = ((c1 * (xx[q] - xx[q - 1]) + c2 * (xx[q + 1] - xx[q - 2])) / 2.0f +
                      (c1 * (xz[q] - xz[q - sz]) + c2 * (xz[q + sz] - xz[q - 2 * sz])) / 2.0f) - xy[q - sy];
   av = c.damp_s * (pdx - fl.avdx[p]);
   rf = pdx + av;
   fl.RFx[p] = rf;
-  fl.tx[p] = -rf - ((0.5f * dd) * dht) * f.u[q];
+  fl.tx[p] = -rf - ((0.5f * dd) * dht) * fn.u[q];
   fl.avdx[p] = pdx;
 }
 
@@ -537,16 +567,16 @@ __device__ inline double fvw_strength(double sr, double a, double b, double vw, 
 // X-node state, so both loops of the reference fuse per node.
 // ---------------------------------------------------------------------------
 template <int BY, bool FVW = true>
-__device__ inline void friction_body(const Dims &g, const Coef &c, const Fields &f, const Fault &fl, const FvwScalars &fv, const Ctl *ctl, int gbx, int gby, int lx, int ly) {
+__device__ inline void friction_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, const Fault &fl, const FvwScalars &fv, const Ctl *ctl, int gbx, int gby, int lx, int ly) {
   const int i = g.nabc + 1 + gbx * BX + lx;
   const int k = g.nabc + 1 + gby * BY + ly;
   if (i > g.nxt - g.nabc || k > g.nzt - g.nfs) return;
   const int p = I2(g, i, k);
   const int q = I3(g, i, g.nyt, k), sz = g.sz;
   const float dt = c.dt;
-  const float time = (float)ctl->it * dt;
+  const float time = (float)(ctl->it + 1) * dt;   // time at the end of this step
   const float rup_tresh = 1.e-3f;
-  const float *u = f.u, *w = f.w;
+  const float *u = fn.u, *w = fn.w;
   // ---- Z-node
   {
     const float tab = fl.tabsZ[p];
@@ -623,8 +653,8 @@ __device__ inline void friction_body(const Dims &g, const Coef &c, const Fields 
 // station samples (:456-467).
 // ---------------------------------------------------------------------------
 template <int BY>
-__device__ inline void outputs_body(const Dims &g, const Fields &f, const Fault &fl, const FvwScalars &fv, const Outputs &o, Ctl *ctl, int gbx, int gby, int lx, int ly) {
-  const int it = ctl->it;
+__device__ inline void outputs_body(const Dims &g, const Fields &fo, const Fields &fn, const Fault &fl, const FvwScalars &fv, const Outputs &o, Ctl *ctl, int gbx, int gby, int lx, int ly) {
+  const int it = ctl->it + 1;   // this step, 1-based
   const int slot = (it - 1) % o.ring_steps;
   const int ks = g.nzt - g.nfs;
   const long long slab = (long long)g.nxT * g.nzT;
@@ -633,7 +663,7 @@ __device__ inline void outputs_body(const Dims &g, const Fields &f, const Fault 
   if (i > g.nxt - g.nabc || k > ks) return;
   const int p = I2(g, i, k), n = g.nxt;
   const int q = I3(g, i, g.nyt, k), sz = g.sz;
-  const float *u = f.u;
+  const float *u = fn.u;
   const float *tx = fl.tx, *T0X = fl.T0X;
   float srx, scx, scz;
   if (k < ks) {
@@ -647,7 +677,7 @@ __device__ inline void outputs_body(const Dims &g, const Fields &f, const Fault 
     srx = ((-(2.0f * u[q])) - 2.0f * u[q + 1]) / 4.0f;
     scx = 0.0f;
   }
-  const float srz = -(2.0f * f.w[q]);
+  const float srz = -(2.0f * fn.w[q]);
   if (fv.fvw) {
     scx = fl.schangeX[p];
     scz = fl.schangeZ[p];
@@ -666,22 +696,36 @@ __device__ inline void outputs_body(const Dims &g, const Fields &f, const Fault 
   if (!isfinite(srx) || !isfinite(srz)) atomicOr(&ctl->nonfinite, 1);
 }
 
+// Last stage of a step: the step counter (and with it the buffer parity)
+// advances once every stage of the step has finished.
+__device__ inline void advance_body(Ctl *ctl) { ctl->it += 1; }
+
+// The old (fo) and new (fn) wavefield sets of the step in progress.
+struct FieldPair {
+  const Fields &fo, &fn;
+};
+__device__ inline FieldPair field_pair(const Fields &f0, const Fields &f1, const Ctl *ctl) {
+  return (ctl->it & 1) ? FieldPair{f1, f0} : FieldPair{f0, f1};
+}
+
 // Station samples of one step into the device ring (fd3d_theo.f90:456-467),
 // one thread per station.
-__device__ inline void stations_body(const Dims &g, const Fields &f, const Outputs &o, const Ctl *ctl, int s) {
+__device__ inline void stations_body(const Dims &g, const Fields &fo, const Fields &fn, const Outputs &o,
+                                     const Ctl *ctl, int s) {
+  (void)fo;
   if (s >= o.nsta) return;
-  const int slot = (ctl->it - 1) % o.ring_steps;
+  const int slot = ctl->it % o.ring_steps;
   const int q = I3(g, o.sta[3 * s], o.sta[3 * s + 1], o.sta[3 * s + 2]);
   float *dst = o.seis + ((long long)slot * o.nsta + s) * 3;
-  dst[0] = f.u[q];
-  dst[1] = f.v[q];
-  dst[2] = f.w[q];
+  dst[0] = fn.u[q];
+  dst[1] = fn.v[q];
+  dst[2] = fn.w[q];
 }
 
 // TPV104 smooth nucleation (fd3d_theo.f90:515-542), FP64.
 template <int BY>
 __device__ inline void nucleation_body(const Dims &g, const Coef &c, const Fault &fl, const FvwScalars &fv, const Ctl *ctl, int gbx, int gby, int lx, int ly) {
-  const double time = (double)((float)ctl->it * c.dt);
+  const double time = (double)((float)(ctl->it + 1) * c.dt);
   const double t = time - (double)c.dt / 2.0;
   if (!(t <= fv.TT2)) return;
   const int i = g.nabc - 1 + gbx * BX + lx;

@@ -42,7 +42,8 @@ struct Tile {
 // yz (z), zz (z).
 // ---------------------------------------------------------------------------
 template <int TZ>
-__device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields &f, float *lds, int bx, int bz,
+__device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, float *lds,
+                                      int bx, int bz,
                                       int by, int tx, int tz, int NY) {
   using T = Tile<TZ>;
   const int t = tz * 64 + tx;
@@ -54,7 +55,7 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
   const bool inside = i <= g.nxt && k <= g.nzt;
   const int sy = g.sy, sz = g.sz;
   const float c1 = c.c1, c2 = c.c2;
-  const float *xx = f.xx, *yy = f.yy, *zz = f.zz, *xy = f.xy, *yz = f.yz, *xz = f.xz;
+  const float *xx = fo.xx, *yy = fo.yy, *zz = fo.zz, *xy = fo.xy, *yz = fo.yz, *xz = fo.xz;
   float xym2 = 0, xym1 = 0, xy0 = 0, xyp1 = 0, yym1 = 0, yy0 = 0, yyp1 = 0, yyp2 = 0;
   float yzm2 = 0, yzm1 = 0, yz0 = 0, yzp1 = 0;
   if (inside) {
@@ -73,6 +74,13 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
     float xx0 = 0, xz0 = 0, zz0 = 0;
     const int q = qj + i + (k - 1) * sz;
     if (inside) { xx0 = xx[q]; xz0 = xz[q]; zz0 = zz[q]; }
+    // every load of this row is issued before the stores below: the next
+    // rows of the y windows and the cell's own old velocities
+    const bool adv = j < je && inside;
+    float nxy = 0, nyy = 0, nyz = 0;
+    if (adv) { nxy = xy[q + 2 * sy]; nyy = yy[q + 3 * sy]; nyz = yz[q + 2 * sy]; }
+    float uo = 0, vo = 0, wo = 0, dq = 1;
+    if (active) { uo = fo.u[q]; vo = fo.v[q]; wo = fo.w[q]; dq = fo.d[q]; }
     L[0][a0] = xx0; L[1][a0] = xy0; L[2][a0] = xz0; L[3][a0] = yz0; L[4][a0] = zz0;
     for (int h = t; h < NHTOT; h += 64 * TZ) {
       int fld, e = h;
@@ -95,21 +103,21 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
     if (active) {
       const float *Lxx = L[0], *Lxy = L[1], *Lxz = L[2], *Lyz = L[3], *Lzz = L[4];
       const int W = T::W;
-      const float b = c.dth / f.d[q];
-      f.u[q] = f.u[q] + b * (c1 * (xx0 - Lxx[a0 - 1]) + c2 * (Lxx[a0 + 1] - Lxx[a0 - 2]) +
+      const float b = c.dth / dq;
+      fn.u[q] = uo + b * (c1 * (xx0 - Lxx[a0 - 1]) + c2 * (Lxx[a0 + 1] - Lxx[a0 - 2]) +
                              c1 * (xy0 - xym1) + c2 * (xyp1 - xym2) +
                              c1 * (xz0 - Lxz[a0 - W]) + c2 * (Lxz[a0 + W] - Lxz[a0 - 2 * W]));
-      f.v[q] = f.v[q] + b * (c1 * (Lxy[a0 + 1] - xy0) + c2 * (Lxy[a0 + 2] - Lxy[a0 - 1]) +
+      fn.v[q] = vo + b * (c1 * (Lxy[a0 + 1] - xy0) + c2 * (Lxy[a0 + 2] - Lxy[a0 - 1]) +
                              c1 * (yyp1 - yy0) + c2 * (yyp2 - yym1) +
                              c1 * (yz0 - Lyz[a0 - W]) + c2 * (Lyz[a0 + W] - Lyz[a0 - 2 * W]));
-      f.w[q] = f.w[q] + b * (c1 * (Lxz[a0 + 1] - xz0) + c2 * (Lxz[a0 + 2] - Lxz[a0 - 1]) +
+      fn.w[q] = wo + b * (c1 * (Lxz[a0 + 1] - xz0) + c2 * (Lxz[a0 + 2] - Lxz[a0 - 1]) +
                              c1 * (yz0 - yzm1) + c2 * (yzp1 - yzm2) +
                              c1 * (Lzz[a0 + W] - zz0) + c2 * (Lzz[a0 + 2 * W] - Lzz[a0 - W]));
     }
-    if (j < je && inside) {
-      xym2 = xym1; xym1 = xy0; xy0 = xyp1; xyp1 = xy[q + 2 * sy];
-      yym1 = yy0; yy0 = yyp1; yyp1 = yyp2; yyp2 = yy[q + 3 * sy];
-      yzm2 = yzm1; yzm1 = yz0; yz0 = yzp1; yzp1 = yz[q + 2 * sy];
+    if (adv) {
+      xym2 = xym1; xym1 = xy0; xy0 = xyp1; xyp1 = nxy;
+      yym1 = yy0; yy0 = yyp1; yyp1 = yyp2; yyp2 = nyy;
+      yzm2 = yzm1; yzm1 = yz0; yz0 = yzp1; yzp1 = nyz;
     }
     buf ^= 1;
   }
@@ -120,7 +128,8 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
 // w (j-1..j+2), mu (j, j+1).  LDS x-z plane at row j: u, v, w, mu.
 // ---------------------------------------------------------------------------
 template <int TZ>
-__device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields &f, float *lds, int bx, int bz,
+__device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, float *lds,
+                                      int bx, int bz,
                                       int by, int tx, int tz, int NY) {
   using T = Tile<TZ>;
   const int t = tz * 64 + tx;
@@ -132,7 +141,7 @@ __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields
   const bool inside = i <= g.nxt && k <= g.nzt;
   const int sy = g.sy, sz = g.sz;
   const float c1 = c.c1, c2 = c.c2, dth = c.dth;
-  const float *u = f.u, *v = f.v, *w = f.w, *mu = f.mu;
+  const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fo.mu;
   float vm2 = 0, vm1 = 0, v0 = 0, vp1 = 0, um1 = 0, u0 = 0, up1 = 0, up2 = 0;
   float wm1 = 0, w0 = 0, wp1 = 0, wp2 = 0, m0 = 0, mp1 = 0;
   if (inside) {
@@ -150,6 +159,16 @@ __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields
     const int a0 = T::at(tx, tz);
     const int q = qj + i + (k - 1) * sz;
     L[0][a0] = u0; L[1][a0] = v0; L[2][a0] = w0; L[3][a0] = m0;
+    // every load of this row is issued before the stores below: the next
+    // rows of the y windows, the cell's old stresses, lam and the diagonal mu
+    const bool adv = j < je && inside;
+    float nv = 0, nu = 0, nw = 0, nm = 0;
+    if (adv) { nv = v[q + 2 * sy]; nu = u[q + 3 * sy]; nw = w[q + 3 * sy]; nm = mu[q + 2 * sy]; }
+    float oxx = 0, oyy = 0, ozz = 0, oxy = 0, oxz = 0, oyz = 0, xl = 0, mdiag = 0;
+    if (active) {
+      oxx = fo.xx[q]; oyy = fo.yy[q]; ozz = fo.zz[q]; oxy = fo.xy[q]; oxz = fo.xz[q]; oyz = fo.yz[q];
+      xl = fo.lam[q]; mdiag = mu[q + 1 + sy + sz];
+    }
     for (int h = t; h < 4 * NH; h += 64 * TZ) {
       const int fld = h / NH, e = h - fld * NH;
       int hx, hz;
@@ -165,29 +184,29 @@ __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields
     if (active) {
       const float *Lu = L[0], *Lv = L[1], *Lw = L[2], *Lm = L[3];
       const int W = T::W;
-      const float xl = f.lam[q], xm = m0;
+      const float xm = m0;
       const float a = xl + 2.0f * xm, b = xl;
       const float dux1 = c1 * (Lu[a0 + 1] - u0), dux2 = c2 * (Lu[a0 + 2] - Lu[a0 - 1]);
       const float dvy1 = c1 * (v0 - vm1), dvy2 = c2 * (vp1 - vm2);
       const float dwz1 = c1 * (w0 - Lw[a0 - W]), dwz2 = c2 * (Lw[a0 + W] - Lw[a0 - 2 * W]);
-      f.xx[q] = (f.xx[q] + (dth * a) * (dux1 + dux2)) + (dth * b) * (((dvy1 + dvy2) + dwz1) + dwz2);
-      f.yy[q] = (f.yy[q] + (dth * a) * (dvy1 + dvy2)) + (dth * b) * (((dux1 + dux2) + dwz1) + dwz2);
-      f.zz[q] = (f.zz[q] + (dth * a) * (dwz1 + dwz2)) + (dth * b) * (((dux1 + dux2) + dvy1) + dvy2);
+      fn.xx[q] = (oxx + (dth * a) * (dux1 + dux2)) + (dth * b) * (((dvy1 + dvy2) + dwz1) + dwz2);
+      fn.yy[q] = (oyy + (dth * a) * (dvy1 + dvy2)) + (dth * b) * (((dux1 + dux2) + dwz1) + dwz2);
+      fn.zz[q] = (ozz + (dth * a) * (dwz1 + dwz2)) + (dth * b) * (((dux1 + dux2) + dvy1) + dvy2);
       float xmu = 0.5f * (xm + mp1);
-      f.xy[q] = f.xy[q] + (dth * xmu) * (((c1 * (up1 - u0) + c2 * (up2 - um1)) + c1 * (v0 - Lv[a0 - 1])) +
+      fn.xy[q] = oxy + (dth * xmu) * (((c1 * (up1 - u0) + c2 * (up2 - um1)) + c1 * (v0 - Lv[a0 - 1])) +
                                          c2 * (Lv[a0 + 1] - Lv[a0 - 2]));
       xmu = 0.5f * (xm + Lm[a0 + W]);
-      f.xz[q] = f.xz[q] + (dth * xmu) * (((c1 * (Lu[a0 + W] - u0) + c2 * (Lu[a0 + 2 * W] - Lu[a0 - W])) +
+      fn.xz[q] = oxz + (dth * xmu) * (((c1 * (Lu[a0 + W] - u0) + c2 * (Lu[a0 + 2 * W] - Lu[a0 - W])) +
                                           c1 * (w0 - Lw[a0 - 1])) + c2 * (Lw[a0 + 1] - Lw[a0 - 2]));
-      xmu = 0.5f * (xm + mu[q + 1 + sy + sz]);
-      f.yz[q] = f.yz[q] + (dth * xmu) * (((c1 * (Lv[a0 + W] - v0) + c2 * (Lv[a0 + 2 * W] - Lv[a0 - W])) +
+      xmu = 0.5f * (xm + mdiag);
+      fn.yz[q] = oyz + (dth * xmu) * (((c1 * (Lv[a0 + W] - v0) + c2 * (Lv[a0 + 2 * W] - Lv[a0 - W])) +
                                           c1 * (wp1 - w0)) + c2 * (wp2 - wm1));
     }
-    if (j < je && inside) {
-      vm2 = vm1; vm1 = v0; v0 = vp1; vp1 = v[q + 2 * sy];
-      um1 = u0; u0 = up1; up1 = up2; up2 = u[q + 3 * sy];
-      wm1 = w0; w0 = wp1; wp1 = wp2; wp2 = w[q + 3 * sy];
-      m0 = mp1; mp1 = mu[q + 2 * sy];
+    if (adv) {
+      vm2 = vm1; vm1 = v0; v0 = vp1; vp1 = nv;
+      um1 = u0; u0 = up1; up1 = up2; up2 = nu;
+      wm1 = w0; w0 = wp1; wp1 = wp2; wp2 = nw;
+      m0 = mp1; mp1 = nm;
     }
     buf ^= 1;
   }

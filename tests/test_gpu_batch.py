@@ -47,7 +47,7 @@ def mix():
 def stage_list(case):
     """Stages of one step (the kernel sequence of a single problem)."""
     st = ["vel_interior", "vel_fault", "pml_vel", "str_interior", "str_fault", "pml_str", "traction",
-          "tabs", "friction", "outputs"]
+          "tabs", "friction", "outputs", "advance"]
     if not case.fspace:
         st += ["fuvw_uv", "fuvw_w", "fres", "traction_mirror"]
     if case.friction == "fvw":
