@@ -243,7 +243,8 @@ void launch_stage_rt(int s, int n, hipStream_t st, const BatchArgs &A, int lo, i
     case 12: launch_stage<TZ, FVW, 12>(n, st, A, lo, hi, i0); break;
     case 13: launch_stage<TZ, FVW, 13>(n, st, A, lo, hi, i0); break;
     case 14: launch_stage<TZ, FVW, 14>(n, st, A, lo, hi, i0); break;
-    default: launch_stage<TZ, FVW, 15>(n, st, A, lo, hi, i0); break;
+    case 15: launch_stage<TZ, FVW, 15>(n, st, A, lo, hi, i0); break;
+    default: launch_stage<TZ, FVW, 16>(n, st, A, lo, hi, i0); break;
   }
 }
 
@@ -328,8 +329,21 @@ BatchDomain describe(const fd3d_handle *h, int tz) {
   items[K_NUCLEATION] = D.nux * cdiv(nzn, tz);
   items[K_ADVANCE] = 1;
   for (int s = 0; s < K_COUNT; ++s) { D.n[s] = 0; D.dep[s] = -1; }
-  int prev = -1;
+  // the batch runs the separate interior sweeps (its work items are tiles
+  // of one sweep), whatever form the problem uses alone
+  std::vector<KernelId> seq;
   for (KernelId id : h->seq) {
+    if (id == K_FUSED) { seq.push_back(K_VEL_INTERIOR); continue; }
+    seq.push_back(id);
+  }
+  {
+    auto it = std::find(seq.begin(), seq.end(), K_STR_FAULT);
+    if (std::find(seq.begin(), seq.end(), K_STR_INTERIOR) == seq.end() && it != seq.end())
+      seq.insert(it, K_STR_INTERIOR);
+  }
+  std::sort(seq.begin(), seq.end());
+  int prev = -1;
+  for (KernelId id : seq) {
     if (items[id] <= 0) continue;
     D.n[id] = items[id];
     D.dep[id] = prev;

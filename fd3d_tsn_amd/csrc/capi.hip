@@ -280,6 +280,30 @@ hipError_t owner_sync(fd3d_handle *h) {
 
 int check_ctl(fd3d_handle *h) { return fd3d_handle_check_ctl(h, h->stream); }
 
+// Kernel sequence of one step (fd3d_theo.f90:208-542).  The velocity
+// kernels read only the old wavefield set and the stress kernels only the
+// new velocities and old stresses, so vel_fault and pml_vel may run before
+// the fused interior sweep, which needs their new velocities.
+void build_seq(fd3d_handle *h) {
+  const Problem &p = h->p;
+  const bool fused = (p.tune.mode & 4) && p.g.fspace;
+  h->seq.clear();
+  if (fused) {
+    h->seq = {K_VEL_FAULT, K_PML_VEL, K_FUSED};
+  } else {
+    h->seq = {K_VEL_INTERIOR, K_VEL_FAULT, K_PML_VEL};
+    if (!p.g.fspace) { h->seq.push_back(K_FUVW_UV); h->seq.push_back(K_FUVW_W); }
+    h->seq.push_back(K_STR_INTERIOR);
+  }
+  h->seq.push_back(K_STR_FAULT); h->seq.push_back(K_PML_STR);
+  if (!p.g.fspace) h->seq.push_back(K_FRES);
+  h->seq.push_back(K_TRACTION);
+  if (!p.g.fspace) h->seq.push_back(K_TRACTION_MIRROR);
+  h->seq.push_back(K_TABS); h->seq.push_back(K_FRICTION); h->seq.push_back(K_OUTPUTS);
+  if (p.fv.nucleation) h->seq.push_back(K_NUCLEATION);
+  h->seq.push_back(K_ADVANCE);
+}
+
 }  // namespace
 
 int fd3d_handle_check_ctl(fd3d_handle *h, hipStream_t s) {
@@ -377,16 +401,7 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
   fv.fw = cfg->fw; fv.hx0 = cfg->hx0; fv.hz0 = cfg->hz0; fv.perturb = cfg->perturb; fv.RR2 = cfg->RR2;
   fv.TT2 = cfg->TT2; fv.strinixI = cfg->strinixI; fv.newton_max = h->cfg.newton_max;
   fv.fvw = cfg->friction == FD3D_FVW; fv.nucleation = fv.fvw && cfg->nucleation;
-  // kernel sequence of one step (fd3d_theo.f90:208-542)
-  h->seq = {K_VEL_INTERIOR, K_VEL_FAULT, K_PML_VEL};
-  if (!p.g.fspace) { h->seq.push_back(K_FUVW_UV); h->seq.push_back(K_FUVW_W); }
-  h->seq.push_back(K_STR_INTERIOR); h->seq.push_back(K_STR_FAULT); h->seq.push_back(K_PML_STR);
-  if (!p.g.fspace) h->seq.push_back(K_FRES);
-  h->seq.push_back(K_TRACTION);
-  if (!p.g.fspace) h->seq.push_back(K_TRACTION_MIRROR);
-  h->seq.push_back(K_TABS); h->seq.push_back(K_FRICTION); h->seq.push_back(K_OUTPUTS);
-  if (fv.nucleation) h->seq.push_back(K_NUCLEATION);
-  h->seq.push_back(K_ADVANCE);
+  build_seq(h);
   if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(FD3D_E_HIP);
   *out = h;
   return 0;
@@ -673,16 +688,20 @@ int fd3d_upload_field(fd3d_handle *h, int which, const float *in) {
 int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str) {
   if (!h) return FD3D_E_INVALID;
   auto ok_tz = [](int tz) { return tz == 4 || tz == 8 || tz == 16; };
-  if (mode < 0 || mode > 3) return fail(h, FD3D_E_INVALID, "tuning mode must be 0..3");
+  if (mode < 0 || mode > 7) return fail(h, FD3D_E_INVALID, "tuning mode must be 0..7");
+  if ((mode & 4) && !h->p.g.fspace)
+    return fail(h, FD3D_E_INVALID, "the fused interior sweep needs a full-space problem");
+  if ((mode & 4) && ny_vel < 1) return fail(h, FD3D_E_INVALID, "rows per block must be >= 1");
   if ((mode & 1) && (!ok_tz(tz_vel) || !ok_tz(tz_str) || ny_vel < 1 || ny_str < 1))
     return fail(h, FD3D_E_INVALID, "tile height must be 4, 8 or 16 and rows >= 1");
   if (h->batch) return fail(h, FD3D_E_STATE, "remove the problem from its batch first");
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->p.tune.mode = mode;
-  if (mode & 1) {
+  if (mode & 5) {
     h->p.tune.tz_vel = tz_vel; h->p.tune.ny_vel = ny_vel;
     h->p.tune.tz_str = tz_str; h->p.tune.ny_str = ny_str;
   }
+  build_seq(h);
   return 0;
 }
 

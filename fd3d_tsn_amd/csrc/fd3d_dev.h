@@ -99,7 +99,8 @@ struct Outputs {
 // Launch shapes of the interior and PML sweeps; chosen per grid size by
 // the autotuner.  mode bit 0: tiled interior sweeps (interior.hip) instead
 // of one thread per cell; bit 1: z-march PML sweeps (pml.hip) instead of
-// one thread per cell (kernels.hip).
+// one thread per cell (kernels.hip); bit 2: the fused interior sweep
+// (fused.hip, full space only; rows per block from ny_vel).
 struct Tuning {
   int mode;
   int tz_vel, ny_vel;   // velocity: x-z tile height (4, 8, 16), rows per block along y
@@ -120,6 +121,7 @@ struct Problem {
 
 enum KernelId {
   K_VEL_INTERIOR = 0, K_VEL_FAULT, K_PML_VEL, K_FUVW_UV, K_FUVW_W,
+  K_FUSED,             // interior velocities and stresses in one pass (full space)
   K_STR_INTERIOR, K_STR_FAULT, K_PML_STR, K_FRES,
   K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_NUCLEATION, K_ADVANCE,
   K_COUNT
@@ -130,6 +132,8 @@ extern const char *kKernelNames[K_COUNT];
 hipError_t launch(KernelId id, const Problem &p, hipStream_t s);
 // Tiled interior sweep (velocity or stress), interior.hip.
 hipError_t launch_interior(bool stress, const Problem &p, hipStream_t s);
+// Fused interior velocity + stress sweep, fused.hip.
+hipError_t launch_fused(const Problem &p, hipStream_t s);
 // z-march PML sweep (velocity or stress), pml.hip.
 hipError_t launch_pml_zm(bool stress, const Problem &p, hipStream_t s);
 // Fill helpers used by the prologue.

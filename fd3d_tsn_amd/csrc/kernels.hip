@@ -16,7 +16,7 @@
 namespace fd3d {
 
 const char *kKernelNames[K_COUNT] = {
-    "vel_interior", "vel_fault", "pml_vel", "fuvw_uv", "fuvw_w",
+    "vel_interior", "vel_fault", "pml_vel", "fuvw_uv", "fuvw_w", "fused_interior",
     "str_interior", "str_fault", "pml_str", "fres",
     "traction", "traction_mirror", "tabs", "friction", "outputs", "nucleation", "advance"};
 
@@ -189,6 +189,8 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
     case K_ADVANCE:
       hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, p.ctl);
       break;
+    case K_FUSED:
+      return launch_fused(p, s);
     default:
       return hipErrorInvalidValue;
   }

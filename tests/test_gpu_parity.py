@@ -161,3 +161,47 @@ def test_launch_shapes_bit_exact(tuning):
         assert np.array_equal(rg[n].astype(np.float64), ro[n]), n
     for n in capi.FIELDS:
         assert np.array_equal(fields[n].astype(np.float64), orc.field(n)), n
+
+
+FUSED = [("fs_ny3", dict(nabc=5, n=(70, 17, 21), seed=33), 3),
+         ("fs_ny1", dict(nabc=4, n=(37, 13, 30), seed=34), 1),
+         ("fs_ny64", dict(nabc=6, n=(130, 20, 14), seed=35), 64),
+         ("fs_tallz", dict(nabc=4, n=(24, 10, 50), seed=36), 4)]
+
+
+@pytest.mark.parametrize("name,kw,ny", FUSED, ids=[f[0] for f in FUSED])
+def test_fused_interior_bit_exact(name, kw, ny):
+    """The fused velocity + stress sweep (full space) gives the oracle's
+    wavefield and outputs bit for bit: partial tiles in x and z, chunks of
+    1 row up to the whole y range, the near-fault rows and PML neighbours
+    read from the other kernels' new velocities.  Starts from a random
+    wavefield so every cell is non-zero from the first step."""
+    case = C.tiny(friction="sw", fspace=True, ntfd=12, **kw)
+    say(f"[parity] fused sweep {name} ny={ny} grid {case.nxt}x{case.nyt}x{case.nzt}")
+    rng = np.random.Generator(np.random.PCG64(7))
+    init = {n: (rng.standard_normal((case.nxt, case.nyt, case.nzt)) *
+                (1e6 if n in ("xx", "yy", "zz", "xy", "yz", "xz") else 1e-2)).astype(np.float32)
+            for n in capi.FIELDS}
+    orc = O.Oracle(case, O.F32)
+    for n, a in init.items():
+        orc.set_field(n, a)
+    ro = orc.step(case.ntfd)
+    with capi.Solver(case) as s:
+        for n, a in init.items():
+            s.set_field(n, a)
+        s.set_tuning(5, 16, ny, 16, ny)
+        assert s.get_tuning()[0] == 5
+        rg = s.step(case.ntfd)
+        fields = {n: s.field(n) for n in capi.FIELDS}
+    for n in OUTS:
+        assert np.array_equal(rg[n].astype(np.float64), ro[n]), n
+    for n in capi.FIELDS:
+        bad = np.argwhere(fields[n].astype(np.float64) != orc.field(n))
+        assert bad.size == 0, f"field {n}: {len(bad)} mismatches, first (0-based) {bad[0].tolist()}"
+
+
+def test_fused_rejected_with_free_surface():
+    case = C.tiny(friction="sw", fspace=False, ntfd=2, seed=37)
+    with capi.Solver(case) as s:
+        with pytest.raises(capi.FD3DError):
+            s.set_tuning(5, 16, 8, 16, 8)
