@@ -94,6 +94,9 @@ def load(path: Optional[str] = None):
     L.fd3d_last_error.argtypes = [H]
     L.fd3d_destroy.argtypes = [H]
     L.fd3d_read_outputs.argtypes = [H, ctypes.c_int, _FP, _FP]
+    L.fd3d_set_source_bins.argtypes = [H, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_int]
+    L.fd3d_download_source_bins.argtypes = [H, _FP, _FP, _FP]
     L.fd3d_device_memory.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     B = ctypes.c_void_p
     L.fd3d_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(B)]
@@ -266,6 +269,26 @@ class Solver:
         res = {name: out[:, o].transpose(0, 2, 1) for o, name in enumerate(OUT_NAMES)}
         res["seis"] = seis[:, :nsta, :]
         return res
+
+    # ------------------------------------------------------ inversion bins
+    def set_source_bins(self, NL: int = 0, NW: int = 0, dL: float = 0.0, dW: float = 0.0, dtseis: float = 1.0,
+                        nbins: Optional[int] = None):
+        """Accumulate MSRX/MSRZ (NL x NW patches of dL x dW metres) and the
+        moment rate in bins of dtseis seconds, as the reference's time loop
+        does for the inversion (fd3d_theo.f90:560-578).  Re-initialises the
+        state (the bins start from zero with the wavefield)."""
+        self._bins = (NL, NW, nbins or self.case.ntfd)
+        self._chk(self.L.fd3d_set_source_bins(self.h, NL, NW, dL, dW, dtseis, self._bins[2]))
+        self.init()
+
+    def source_bins(self) -> Dict[str, np.ndarray]:
+        NL, NW, nb = self._bins
+        msrx = np.zeros(max(NL * NW * nb, 1), np.float32)
+        msrz = np.zeros(max(NL * NW * nb, 1), np.float32)
+        mr = np.zeros(nb, np.float32)
+        self._chk(self.L.fd3d_download_source_bins(self.h, _fp(msrx), _fp(msrz), _fp(mr)))
+        return {"MSRX": msrx[:NL * NW * nb].reshape(NL * NW, nb), "MSRZ": msrz[:NL * NW * nb].reshape(NL * NW, nb),
+                "MomentRate": mr}
 
     def steps_done(self) -> int:
         return int(self.L.fd3d_steps_done(self.h))

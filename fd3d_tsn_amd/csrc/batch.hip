@@ -137,6 +137,9 @@ __device__ __forceinline__ void run_item(const BatchDomain &D, int stage, int b,
     case K_ADVANCE:
       if (t == 0) advance_body(P.ctl);
       break;
+    case K_RATES:
+      rates_body<BLK>(P.g, P.c, fn, P.out, P.rates, P.ctl, b, t, reinterpret_cast<double *>(lds));
+      break;
     case K_NUCLEATION:
       nucleation_body<TZ>(P.g, P.c, P.fl, P.fv, P.ctl, b % D.nux, b / D.nux, lx, ly);
       break;
@@ -214,7 +217,9 @@ __global__ void __launch_bounds__(64 * TZ) k_batch_persistent(BatchArgs A) {
 // instantiation holds (and allocates registers for) one stage body only.
 template <int TZ, bool FVW, int STAGE>
 __global__ void __launch_bounds__(64 * TZ) k_batch_stage(BatchArgs A, int seg_lo, int seg_hi, int item0) {
-  __shared__ float lds[STAGE == K_VEL_INTERIOR || STAGE == K_STR_INTERIOR ? Tile<TZ>::LDS_FLOATS : 1];
+  __shared__ float lds[STAGE == K_VEL_INTERIOR || STAGE == K_STR_INTERIOR ? Tile<TZ>::LDS_FLOATS
+                       : STAGE == K_RATES                                  ? 2 * 64 * TZ
+                                                                           : 1];
   const int item = item0 + (int)blockIdx.x;
   const int sg = find_seg(A.seg, seg_lo, seg_hi, item);
   run_item<TZ, FVW>(A.dom[A.seg[sg].dom], STAGE, item - A.seg[sg].start, lds, threadIdx.x, threadIdx.y);
@@ -244,7 +249,8 @@ void launch_stage_rt(int s, int n, hipStream_t st, const BatchArgs &A, int lo, i
     case 13: launch_stage<TZ, FVW, 13>(n, st, A, lo, hi, i0); break;
     case 14: launch_stage<TZ, FVW, 14>(n, st, A, lo, hi, i0); break;
     case 15: launch_stage<TZ, FVW, 15>(n, st, A, lo, hi, i0); break;
-    default: launch_stage<TZ, FVW, 16>(n, st, A, lo, hi, i0); break;
+    case 16: launch_stage<TZ, FVW, 16>(n, st, A, lo, hi, i0); break;
+    default: launch_stage<TZ, FVW, 17>(n, st, A, lo, hi, i0); break;
   }
 }
 
@@ -328,6 +334,7 @@ BatchDomain describe(const fd3d_handle *h, int tz) {
   items[K_OUTPUTS] = D.nout + cdiv(h->p.out.nsta, blk);
   items[K_NUCLEATION] = D.nux * cdiv(nzn, tz);
   items[K_ADVANCE] = 1;
+  items[K_RATES] = h->p.rates.on ? 1 + cdiv(h->p.rates.NL * h->p.rates.NW, blk) : 0;
   for (int s = 0; s < K_COUNT; ++s) { D.n[s] = 0; D.dep[s] = -1; }
   // the batch runs the separate interior sweeps (its work items are tiles
   // of one sweep), whatever form the problem uses alone

@@ -300,6 +300,7 @@ void build_seq(fd3d_handle *h) {
   h->seq.push_back(K_TRACTION);
   if (!p.g.fspace) h->seq.push_back(K_TRACTION_MIRROR);
   h->seq.push_back(K_TABS); h->seq.push_back(K_FRICTION); h->seq.push_back(K_OUTPUTS);
+  if (p.rates.on) h->seq.push_back(K_RATES);
   if (p.fv.nucleation) h->seq.push_back(K_NUCLEATION);
   h->seq.push_back(K_ADVANCE);
 }
@@ -517,6 +518,14 @@ int fd3d_init_state(fd3d_handle *h) {
   for (int r = 0; r < p.pml.nreg; ++r) nsplit += (long long)kSplits * p.pml.r[r].ncell;
   HIPCHK(h, hipMemsetAsync(p.pml.splits, 0, (size_t)nsplit * sizeof(float), s));
   HIPCHK(h, hipMemsetAsync(p.ctl, 0, sizeof(Ctl), s));
+  if (p.rates.on) {   // MSRX = MSRZ = MomentRate = 0 (fd3d_theo.f90:67)
+    const size_t nb = (size_t)p.rates.NL * p.rates.NW * p.rates.nbins;
+    if (nb) {
+      HIPCHK(h, hipMemsetAsync(p.rates.msrx, 0, nb * sizeof(float), s));
+      HIPCHK(h, hipMemsetAsync(p.rates.msrz, 0, nb * sizeof(float), s));
+    }
+    HIPCHK(h, hipMemsetAsync(p.rates.mr, 0, (size_t)p.rates.nbins * sizeof(float), s));
+  }
   // interp_fric (fd3d_deriv.f90:3150-3190) on the host, float arithmetic
   std::vector<float> peakX(n2, 0), dynX(n2, 0), DcX(n2, 0), T0X(n2, 0);
   auto I = [&](int i, int k) { return (size_t)(i - 1) + (size_t)nxt * (size_t)(k - 1); };
@@ -683,6 +692,38 @@ int fd3d_upload_field(fd3d_handle *h, int which, const float *in) {
   int rc = upload3(h, field_ptr(h, which, 0), in);
   if (rc == 0) rc = upload3(h, field_ptr(h, which, 1), in);
   return rc;
+}
+
+int fd3d_set_source_bins(fd3d_handle *h, int NL, int NW, float dL, float dW, float dtseis, int nbins) {
+  if (!h) return FD3D_E_INVALID;
+  if (NL < 0 || NW < 0 || nbins < 1 || !(dtseis > 0) || ((NL > 0 || NW > 0) && !(dL > 0 && dW > 0)))
+    return fail(h, FD3D_E_INVALID, "source bins: NL, NW >= 0, dL, dW, dtseis > 0, nbins >= 1");
+  if (h->batch) return fail(h, FD3D_E_STATE, "remove the problem from its batch first");
+  if (h->p.rates.on) return fail(h, FD3D_E_STATE, "source bins are already set");
+  Rates &R = h->p.rates;
+  const size_t nb = (size_t)NL * NW * nbins;
+  int rc;
+  if ((rc = dalloc(h, &R.msrx, std::max<size_t>(nb, 1)))) return rc;
+  if ((rc = dalloc(h, &R.msrz, std::max<size_t>(nb, 1)))) return rc;
+  if ((rc = dalloc(h, &R.mr, (size_t)nbins))) return rc;
+  HIPCHK(h, hipMemset(R.msrx, 0, std::max<size_t>(nb, 1) * sizeof(float)));
+  HIPCHK(h, hipMemset(R.msrz, 0, std::max<size_t>(nb, 1) * sizeof(float)));
+  HIPCHK(h, hipMemset(R.mr, 0, (size_t)nbins * sizeof(float)));
+  R.NL = NL; R.NW = NW; R.nbins = nbins; R.dL = dL; R.dW = dW; R.dtseis = dtseis; R.on = 1;
+  build_seq(h);
+  return 0;
+}
+
+int fd3d_download_source_bins(fd3d_handle *h, float *msrx, float *msrz, float *moment_rate) {
+  if (!h) return FD3D_E_INVALID;
+  const Rates &R = h->p.rates;
+  if (!R.on) return fail(h, FD3D_E_STATE, "fd3d_set_source_bins first");
+  HIPCHK(h, owner_sync(h));
+  const size_t nb = (size_t)R.NL * R.NW * R.nbins;
+  if (msrx && nb) HIPCHK(h, hipMemcpy(msrx, R.msrx, nb * sizeof(float), hipMemcpyDeviceToHost));
+  if (msrz && nb) HIPCHK(h, hipMemcpy(msrz, R.msrz, nb * sizeof(float), hipMemcpyDeviceToHost));
+  if (moment_rate) HIPCHK(h, hipMemcpy(moment_rate, R.mr, (size_t)R.nbins * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str) {

@@ -89,6 +89,16 @@ struct Ctl {
   int pad;
 };
 
+// Source-time bins for dynamic inversion (SlipRates_com, fd3d_init.f90:
+// 116-120; accumulated at fd3d_theo.f90:560-578): mean slip rates over
+// NL x NW fault patches (msrx, msrz, patch-major then bin) and the moment
+// rate, in bins of dtseis seconds.
+struct Rates {
+  float *msrx, *msrz, *mr;
+  int NL, NW, nbins, on;
+  float dL, dW, dtseis;
+};
+
 struct Outputs {
   float *ring;         // [ring][FD3D_NOUT][nxT*nzT]
   float *seis;         // [ring][nsta][3]
@@ -116,6 +126,7 @@ struct Problem {
   FvwScalars fv;
   Pml pml;
   Outputs out;
+  Rates rates;
   Ctl *ctl;
 };
 
@@ -123,7 +134,7 @@ enum KernelId {
   K_VEL_INTERIOR = 0, K_VEL_FAULT, K_PML_VEL, K_FUVW_UV, K_FUVW_W,
   K_FUSED,             // interior velocities and stresses in one pass (full space)
   K_STR_INTERIOR, K_STR_FAULT, K_PML_STR, K_FRES,
-  K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_NUCLEATION, K_ADVANCE,
+  K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_RATES, K_NUCLEATION, K_ADVANCE,
   K_COUNT
 };
 extern const char *kKernelNames[K_COUNT];

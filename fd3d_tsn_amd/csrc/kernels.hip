@@ -18,7 +18,7 @@ namespace fd3d {
 const char *kKernelNames[K_COUNT] = {
     "vel_interior", "vel_fault", "pml_vel", "fuvw_uv", "fuvw_w", "fused_interior",
     "str_interior", "str_fault", "pml_str", "fres",
-    "traction", "traction_mirror", "tabs", "friction", "outputs", "nucleation", "advance"};
+    "traction", "traction_mirror", "tabs", "friction", "outputs", "rates", "nucleation", "advance"};
 
 namespace {
 
@@ -93,6 +93,13 @@ __global__ void __launch_bounds__(256) k_nucleation(Dims g, Coef c, Fault fl, Fv
   nucleation_body<BY>(g, c, fl, fv, ctl, B2);
 }
 __global__ void k_advance(Ctl *ctl) { advance_body(ctl); }
+__global__ void __launch_bounds__(1024) k_rates(Dims g, Coef c, Fields f0, Fields f1, Outputs o, Rates R,
+                                                const Ctl *ctl) {
+  __shared__ double red[1024];
+  PAIR;
+  (void)fo;
+  rates_body<1024>(g, c, fn, o, R, ctl, blockIdx.x, threadIdx.x, red);
+}
 
 #undef PAIR
 #undef B2
@@ -191,6 +198,11 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
       break;
     case K_FUSED:
       return launch_fused(p, s);
+    case K_RATES:
+      if (!p.rates.on) return hipSuccess;
+      hipLaunchKernelGGL(k_rates, dim3(1 + cdiv(p.rates.NL * p.rates.NW, 1024)), dim3(1024), 0, s, g, p.c, p.f[0],
+                         p.f[1], p.out, p.rates, p.ctl);
+      break;
     default:
       return hipErrorInvalidValue;
   }

@@ -696,6 +696,67 @@ __device__ inline void outputs_body(const Dims &g, const Fields &fo, const Field
   if (!isfinite(srx) || !isfinite(srz)) atomicOr(&ctl->nonfinite, 1);
 }
 
+// Source-time bins of this step (fd3d_theo.f90:560-578), after the output
+// slabs of the step are in the ring.  Item 0 adds the step's moment rate
+// to its bin: each term is the reference's float expression, the sum over
+// the fault is a fixed-order tree in double (the reference accumulates in
+// float node by node, so the two agree to rounding, not bit for bit).
+// Items 1.. hold one patch per lane: the patch sum is sequential in the
+// reference's order (x fastest), then added to the bin as the reference
+// does (float sum / double divisor, rounded back to float).  Nodes outside
+// the output slab count as zero, as sliprateoutX/Z are zero there.
+// red: NT doubles of LDS.
+template <int NT>
+__device__ inline void rates_body(const Dims &g, const Coef &c, const Fields &fn, const Outputs &o, const Rates &R,
+                                  const Ctl *ctl, int item, int t, double *red) {
+  const int it1 = ctl->it + 1;   // this step, 1-based
+  const int kbin = (int)(((float)(it1 - 1) * c.dt) / R.dtseis) + 1;
+  if (kbin < 1 || kbin > R.nbins) return;   // the reference reports CHYBA and writes out of range
+  const long long slab = (long long)g.nxT * g.nzT;
+  const float *srx = o.ring + (long long)(ctl->it % o.ring_steps) * 5 * slab;
+  const float *srz = srx + slab;
+  const float per = R.dtseis / c.dt;
+  if (item == 0) {
+    double acc = 0.0;
+    for (long long n = t; n < slab; n += NT) {
+      const int i = g.nabc + 1 + (int)(n % g.nxT), k = g.nabc + 1 + (int)(n / g.nxT);
+      const float sx = srx[n], sz = srz[n];
+      const float m = fn.mu[I3(g, i, g.nyt, k)];
+      acc += (double)((((sqrtf(sx * sx + sz * sz) * m) * c.dh) * c.dh) / per);
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+      if (t < w) red[t] += red[t + w];
+      __syncthreads();
+    }
+    if (t == 0) R.mr[kbin - 1] = (float)((double)R.mr[kbin - 1] + red[0]);
+    return;
+  }
+  const int p = (item - 1) * NT + t;
+  if (p >= R.NL * R.NW) return;
+  const int ip = p % R.NL + 1, jp = p / R.NL + 1;   // 1-based patch indices (do j=1,NW / do i=1,NL)
+  const int jto = max(1, (int)((R.dW / c.dh) * (float)jp)) + 1 + g.nabc;
+  const int jfrom = min(jto, (int)((R.dW / c.dh) * (float)(jp - 1)) + 1) + 1 + g.nabc;
+  const int ifrom = (int)((R.dL / c.dh) * (float)(ip - 1)) + 1 + g.nabc;
+  const int ito = (int)((R.dL / c.dh) * (float)ip) + g.nabc;
+  float sx = 0.0f, sz = 0.0f;
+  for (int kk = jfrom; kk <= jto; ++kk)
+    for (int ii = ifrom; ii <= ito; ++ii) {
+      float vx = 0.0f, vz = 0.0f;
+      if (ii >= g.nabc + 1 && ii <= g.nxt - g.nabc && kk >= g.nabc + 1 && kk <= g.nzt - g.nfs) {
+        const long long e = (long long)(ii - g.nabc - 1) + (long long)g.nxT * (kk - g.nabc - 1);
+        vx = srx[e]; vz = srz[e];
+      }
+      sx = sx + vx;
+      sz = sz + vz;
+    }
+  const double div = (double)((float)((ito - ifrom + 1) * (jto - jfrom + 1)) * per);
+  const long long q = (long long)(p) * R.nbins + (kbin - 1);
+  R.msrx[q] = (float)((double)R.msrx[q] + (double)sx / div);
+  R.msrz[q] = (float)((double)R.msrz[q] + (double)sz / div);
+}
+
 // Last stage of a step: the step counter (and with it the buffer parity)
 // advances once every stage of the step has finished.
 __device__ inline void advance_body(Ctl *ctl) { ctl->it += 1; }

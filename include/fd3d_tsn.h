@@ -154,10 +154,28 @@ int fd3d_download_final(fd3d_handle *h, float *out7);
 int fd3d_download_field(fd3d_handle *h, int which, float *out);
 int fd3d_upload_field(fd3d_handle *h, int which, const float *in);
 
+/* Source-time bins for dynamic inversion, accumulated on the device every
+ * step as fd3d_theo.f90:560-578 does into SlipRates_com (MSRX, MSRZ,
+ * MomentRate; allocated in inversion_init, fd3d_init.f90:786-800):
+ *   bin k = int((it-1)*dt/dtseis)+1 of nbins (the reference: dtseis = 1 s,
+ *   nbins = nSR = ntfd);
+ *   msrx/msrz[(patch)*nbins + k-1] += mean slip rate over patch (i, j) of an
+ *   NL x NW grid of dL x dW patches, times dt/dtseis;
+ *   moment_rate[k-1] += sum over the fault of |slip rate| * mu * dh^2 * dt/dtseis.
+ * Call before fd3d_init_state (which zeroes them); NL = NW = 0 keeps only
+ * the moment rate.  The patch sums follow the reference's float order; the
+ * moment-rate sum over the fault is a fixed-order double-precision tree
+ * (the reference adds node by node in float). */
+int fd3d_set_source_bins(fd3d_handle *h, int NL, int NW, float dL, float dW, float dtseis, int nbins);
+/* msrx, msrz: NL*NW*nbins floats each (or NULL); moment_rate: nbins floats (or NULL). */
+int fd3d_download_source_bins(fd3d_handle *h, float *msrx, float *msrz, float *moment_rate);
+
 /* Launch shapes of the 3D sweeps.  mode bit 0: interior sweeps where a
  * block owns a 64 x tz tile of the x-z plane and marches ny rows along y
  * (tz in {4, 8, 16}, ny >= 1), else one thread per cell; mode bit 1: PML
- * sweeps marching z in register windows, else one thread per cell.  fd3d_create picks a default from
+ * sweeps marching z in register windows, else one thread per cell; mode
+ * bit 2: one fused velocity + stress pass over the interior (full space
+ * only, ny_vel rows per block).  fd3d_create picks a default from
  * the grid size; fd3d_tsn_amd/autotune.py measures and caches the best.
  * Results do not depend on the shape (bit-identical). */
 int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str);

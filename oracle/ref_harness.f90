@@ -9,7 +9,10 @@
 ! by default this program reads the fault fields from 'fault_in.bin' after
 ! fd3d_init()/inversion_init() (dynamicsolver.f90:35-36) and then calls
 ! fd3d() exactly as dynamicsolver.f90:62-66 does.  After the loop it
-! writes the 2D result arrays as one raw stream, 'result/final.bin'.
+! writes the 2D result arrays as one raw stream, 'result/final.bin', and
+! the inversion accumulators MSRX, MSRZ, MomentRate, 'result/rates.bin'.
+! Optional file 'rates_in.txt' (NL NW dL dW dtseis) sets the patch grid
+! and bin width of those accumulators before inversion_init allocates them.
 !
 ! Optional file 'harness_setup.txt' (first word):
 !   file       read fault_in.bin (default)
@@ -35,7 +38,8 @@
       IMPLICIT NONE
       integer :: u, ios
       character(len=32) :: setup, action
-      logical :: have
+      logical :: have, rates
+      real :: dts
 
       setup = 'file'
       action = ''
@@ -47,7 +51,16 @@
       endif
 
       call fd3d_init()
+      ! optional source-time bins of the inversion (SlipRates_com): the
+      ! reference leaves NL = NW = 0 and dtseis = 1 (inversion_init)
+      inquire(file='rates_in.txt', exist=rates)
+      if (rates) then
+        open(newunit=u, file='rates_in.txt', status='old')
+        read(u, *) NL, NW, dL, dW, dts
+        close(u)
+      endif
       call inversion_init()
+      if (rates) dtseis = dts
 
       select case (trim(setup))
       case ('file')
@@ -89,6 +102,9 @@
       ioutput = 1
       call fd3d()
 
+      open(newunit=u, file='result/rates.bin', form='unformatted', access='stream', status='replace')
+      write(u) MSRX, MSRZ, MomentRate
+      close(u)
       open(newunit=u, file='result/final.bin', form='unformatted', access='stream', status='replace')
       write(u) ruptime, rise, sliptime, slipX, slipZ, schangeX, schangeZ
       close(u)

@@ -128,8 +128,31 @@ def example_fixtures(manifest):
         shutil.rmtree(wd)
 
 
+def rates_cases():
+    """Inversion accumulators (MSRX, MSRZ, MomentRate) of the reference's time
+    loop, with a 4 x 2 patch grid and 0.04 s bins set through the harness."""
+    c = C.tiny("sw", True, ntfd=120, seed=61)
+    nxT, nzT = c.fault_shape
+    return {"rates_sw_fs": (c, (4, 2, nxT * c.dh / 4, nzT * c.dh / 2, 0.04))}
+
+
+def rates_fixtures(manifest):
+    for name, (case, rates) in rates_cases().items():
+        print(f"[golden] {name}: inversion accumulators {rates}", flush=True)
+        res, _w, _l = refrun.run(case, rates=rates)
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, MSRX=res["MSRX"].astype(np.float32), MSRZ=res["MSRZ"].astype(np.float32),
+                            MomentRate=res["MomentRate"].astype(np.float32))
+        cfg = dataclasses.asdict(case)
+        cfg["stations"] = [list(s) for s in case.stations]
+        manifest[name] = {"case": cfg, "rates": list(rates), "file": os.path.basename(path),
+                          "reference_build": os.path.basename(refrun.variant(case)),
+                          "sha256": hashlib.sha256(open(path, "rb").read()).hexdigest()}
+
+
 def main():
     manifest = {}
+    rates_fixtures(manifest)
     example_fixtures(manifest)
     driver_fixtures(manifest)
     for name, case in golden_cases().items():

@@ -97,13 +97,19 @@ def _big_stack():
 
 
 def run(case: C.Case, fault=None, fvw_scalars=None, fast: bool = False, timeout: float = 3600,
-        workdir: Optional[str] = None, keep_outputs: bool = True, cores: Optional[str] = None):
-    """Run the reference; returns (outputs, wall seconds, loop CPU seconds)."""
+        workdir: Optional[str] = None, keep_outputs: bool = True, cores: Optional[str] = None,
+        rates: Optional[tuple] = None):
+    """Run the reference; returns (outputs, wall seconds, loop CPU seconds).
+    rates: (NL, NW, dL, dW, dtseis) for the inversion accumulators, which
+    come back as outputs 'MSRX', 'MSRZ' (NL*NW x ntfd) and 'MomentRate'."""
     exe = variant(case, fast)
     if not os.access(exe, os.X_OK):
         raise FileNotFoundError(exe)
     wd = workdir or tempfile.mkdtemp(prefix="fd3dref_", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
     write_inputs(case, wd, fault, fvw_scalars)
+    if rates is not None:
+        with open(os.path.join(wd, "rates_in.txt"), "w") as f:
+            f.write(" ".join(repr(x) for x in rates) + "\n")
     cmd = [exe]
     if cores is not None:
         cmd = ["taskset", "-c", cores, exe]
@@ -117,6 +123,14 @@ def run(case: C.Case, fault=None, fvw_scalars=None, fast: bool = False, timeout:
         if "CPU TIME OF TIME LOOP" in line:
             loop = float(line.split(":")[-1])
     res = read_outputs(case, wd) if keep_outputs else {}
+    if keep_outputs and rates is not None:
+        dt = np.float64 if _f64(case) else np.float32
+        r = np.fromfile(os.path.join(wd, "result", "rates.bin"), dtype=dt)
+        nlw = int(rates[0]) * int(rates[1])
+        n = case.ntfd
+        res["MSRX"] = r[:nlw * n].reshape(nlw, n)
+        res["MSRZ"] = r[nlw * n:2 * nlw * n].reshape(nlw, n)
+        res["MomentRate"] = r[2 * nlw * n:2 * nlw * n + n]
     if workdir is None:
         subprocess.run(["rm", "-rf", wd])
     res["_stdout"] = p.stdout
