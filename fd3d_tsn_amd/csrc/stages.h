@@ -699,13 +699,13 @@ __device__ inline void outputs_body(const Dims &g, const Fields &fo, const Field
 // Source-time bins of this step (fd3d_theo.f90:560-578), after the output
 // slabs of the step are in the ring.  Item 0 adds the step's moment rate
 // to its bin: each term is the reference's float expression, the sum over
-// the fault is a fixed-order tree in double (the reference accumulates in
+// the fault is a fixed-order tree in double (independent of the block size) (the reference accumulates in
 // float node by node, so the two agree to rounding, not bit for bit).
 // Items 1.. hold one patch per lane: the patch sum is sequential in the
 // reference's order (x fastest), then added to the bin as the reference
 // does (float sum / double divisor, rounded back to float).  Nodes outside
 // the output slab count as zero, as sliprateoutX/Z are zero there.
-// red: NT doubles of LDS.
+// red: 256 doubles of LDS.
 template <int NT>
 __device__ inline void rates_body(const Dims &g, const Coef &c, const Fields &fn, const Outputs &o, const Rates &R,
                                   const Ctl *ctl, int item, int t, double *red) {
@@ -717,16 +717,21 @@ __device__ inline void rates_body(const Dims &g, const Coef &c, const Fields &fn
   const float *srz = srx + slab;
   const float per = R.dtseis / c.dt;
   if (item == 0) {
-    double acc = 0.0;
-    for (long long n = t; n < slab; n += NT) {
-      const int i = g.nabc + 1 + (int)(n % g.nxT), k = g.nabc + 1 + (int)(n / g.nxT);
-      const float sx = srx[n], sz = srz[n];
-      const float m = fn.mu[I3(g, i, g.nyt, k)];
-      acc += (double)((((sqrtf(sx * sx + sz * sz) * m) * c.dh) * c.dh) / per);
+    // 256 partial sums (node n goes to partial n % 256) and a fixed tree:
+    // the same result whatever the block size NT (>= 256) that runs it
+    static_assert(NT >= 256, "rates_body needs at least 256 lanes");
+    if (t < 256) {
+      double acc = 0.0;
+      for (long long n = t; n < slab; n += 256) {
+        const int i = g.nabc + 1 + (int)(n % g.nxT), k = g.nabc + 1 + (int)(n / g.nxT);
+        const float sx = srx[n], sz = srz[n];
+        const float m = fn.mu[I3(g, i, g.nyt, k)];
+        acc += (double)((((sqrtf(sx * sx + sz * sz) * m) * c.dh) * c.dh) / per);
+      }
+      red[t] = acc;
     }
-    red[t] = acc;
     __syncthreads();
-    for (int w = NT / 2; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
       if (t < w) red[t] += red[t + w];
       __syncthreads();
     }

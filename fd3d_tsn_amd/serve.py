@@ -14,8 +14,9 @@ arrive as a stream and share one GPU:
 * while a chunk runs on the GPU, the host prepares the next requests
   (inputs, device allocation, upload, prologue) on their own streams; they
   join at the next chunk boundary, when finished problems leave;
-* each result (the final 2D arrays of fd3d(), fd3d_theo.f90:604-658) is
-  bit-identical to the same request solved alone.
+* each result (the final 2D arrays of fd3d(), fd3d_theo.f90:604-658, and
+  on request the inversion's patch slip rates and moment rate,
+  fd3d_theo.f90:560-578) is bit-identical to the same request solved alone.
 
 :func:`run_plain` serves the same stream back to back, one request at a
 time through fd3d_step, as the baseline.
@@ -39,6 +40,9 @@ class Request:
     rid: int
     arrival: float          # seconds after the start of the stream
     case: C.Case
+    # inversion products to accumulate on the device: (NL, NW, dL, dW, dtseis)
+    # for fd3d_set_source_bins, or None for the final arrays only
+    bins: Optional[tuple] = None
 
     @property
     def steps(self) -> int:
@@ -54,6 +58,7 @@ class Result:
     steps: int
     cells: int
     final: Dict[str, np.ndarray] = field(repr=False, default_factory=dict)
+    bins: Dict[str, np.ndarray] = field(repr=False, default_factory=dict)
 
     @property
     def latency(self) -> float:
@@ -158,6 +163,8 @@ class Server:
             while queue and len(active) + len(ready) < self.max_slots and self._fits(queue[0], 0):
                 r = queue.popleft()
                 s = capi.Solver(r.case)
+                if r.bins is not None:
+                    s.set_source_bins(*r.bins, nbins=r.steps)
                 autotune.apply(s, self.tz)       # rows per tile for this size class
                 ready.append({"req": r, "solver": s, "left": r.steps, "admitted": clock() - t0})
             if queue and not active and not ready:
@@ -175,10 +182,11 @@ class Server:
                     s = a["solver"]
                     self.batch.remove(s)
                     fin = s.final() if self.keep_final else {}
-                    s.close()
                     r = a["req"]
+                    bins = s.source_bins() if r.bins is not None else {}
+                    s.close()
                     results.append(Result(r.rid, r.arrival, a["admitted"], clock() - t0, r.steps, r.case.cells,
-                                          fin))
+                                          fin, bins))
                     active.remove(a)
                     self.log(f"[serve] request {r.rid} done: latency {results[-1].latency:.3f} s, "
                              f"{len(active)} active, {len(queue)} queued")
@@ -199,14 +207,17 @@ def run_plain(requests: List[Request], keep_final: bool = True, clock=time.perf_
     for r in sorted(requests, key=lambda q: q.arrival):
         t1 = clock()
         with capi.Solver(r.case) as s:
+            if r.bins is not None:
+                s.set_source_bins(*r.bins, nbins=r.steps)
             autotune.apply(s)
             s.step(r.steps, outputs=False)
             s.sync()
             fin = s.final() if keep_final else {}
+            bins = s.source_bins() if r.bins is not None else {}
         service = clock() - t1
         start = max(free_at, r.arrival)
         free_at = start + service
-        results.append(Result(r.rid, r.arrival, start, free_at, r.steps, r.case.cells, fin))
+        results.append(Result(r.rid, r.arrival, start, free_at, r.steps, r.case.cells, fin, bins))
         if log:
             log(f"[plain] request {r.rid}: {service:.3f} s")
     return results

@@ -32,7 +32,12 @@ def requests():
         (0.05, C.tiny("fvw", True, seed=55, ntfd=32)),
         (0.30, C.tiny("sw", True, seed=56, ntfd=20)),
     ]
-    return [serve.Request(q, t, c) for q, (t, c) in enumerate(specs)]
+    reqs = [serve.Request(q, t, c) for q, (t, c) in enumerate(specs)]
+    # two requests also accumulate the inversion bins
+    for r in (reqs[0], reqs[3]):
+        nxT, nzT = r.case.fault_shape
+        r.bins = (2, 2, nxT * r.case.dh / 2, nzT * r.case.dh / 2, 0.016)
+    return reqs
 
 
 def test_served_equals_alone():
@@ -50,6 +55,9 @@ def test_served_equals_alone():
         ref = plain[r.rid]
         for k, v in ref.final.items():
             assert np.array_equal(r.final[k], v), f"request {r.rid} final {k}"
+        assert sorted(r.bins) == sorted(ref.bins)
+        for k, v in ref.bins.items():
+            assert np.array_equal(r.bins[k], v), f"request {r.rid} bins {k}"
         assert r.finished >= r.admitted >= 0.0 and r.latency > 0.0
     stats = serve.summary(got, srv.makespan)
     say(f"[serve] {stats}")
