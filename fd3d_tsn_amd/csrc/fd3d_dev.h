@@ -68,6 +68,23 @@ struct PmlRegion {
   int prof_off;            // offset of the profile block in the profile pool
 };
 
+// Logical block id of hardware block pid when blocks are dealt round-robin
+// to 8 XCDs: XCD x gets the contiguous logical range [x*N/8, (x+1)*N/8),
+// so blocks that work on neighbouring data share that XCD's L2.
+__device__ inline int xcd_remap(int pid, int N) {
+  const int q = N >> 3, r = N & 7;
+  const int xcd = pid & 7, loc = pid >> 3;
+  return xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+}
+
+// Non-temporal access for values a sweep touches once: the new wavefield
+// values it writes and the old centre values it reads.  Their lines then do
+// not displace the stencil operands that neighbouring cells re-read from L2
+// and the last-level cache, and the next sweep does not start behind the
+// write-back of dirty lines.
+__device__ __forceinline__ void st_nt(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ float ld_nt(const float *p) { return __builtin_nontemporal_load(p); }
+
 constexpr int kMaxRegions = 5;
 constexpr int kSplits = 24;
 constexpr int kPmlBlock = 256;

@@ -232,7 +232,7 @@ __device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &
   pt = su3 * pamz + (dtd * (xz0 - xzmz)) / dh;
   s3 = pt / papz;
   s[0 * nc] = s1; s[1 * nc] = s2; s[2 * nc] = s3;
-  fn.u[q] = (s1 + s2) + s3;
+  st_nt(&fn.u[q], (s1 + s2) + s3);
   // w (xS, y, z profiles)
   pt = sw1 * pamxS + (dtd * (xzp - xz0)) / dh;
   s1 = pt / papxS;
@@ -241,7 +241,7 @@ __device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &
   pt = sw3 * pamz + (dtd * (zzp - zz0)) / dh;
   s3 = pt / papz;
   s[6 * nc] = s1; s[7 * nc] = s2; s[8 * nc] = s3;
-  fn.w[q] = (s1 + s2) + s3;
+  st_nt(&fn.w[q], (s1 + s2) + s3);
   if (hasv) {
     pt = sv1 * pamxS + (dtd * (xyp - xy0)) / dh;
     s1 = pt / papxS;
@@ -250,7 +250,7 @@ __device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &
     pt = sv3 * pamz + (dtd * (yz0 - yzmz)) / dh;
     s3 = pt / papz;
     s[3 * nc] = s1; s[4 * nc] = s2; s[5 * nc] = s3;
-    fn.v[q] = (s1 + s2) + s3;
+    st_nt(&fn.v[q], (s1 + s2) + s3);
   }
 }
 
@@ -430,30 +430,30 @@ __device__ inline void pml_str_body(const Dims &g, const Coef &c, const Fields &
   pt = sxx2 * ay + ((dt * b) * diff2) / dh; s2 = pt / by;
   pt = sxx3 * az + ((dt * b) * diff3) / dh; s3 = pt / bz;
   s[9 * nc] = s1; s[10 * nc] = s2; s[11 * nc] = s3;
-  fn.xx[q] = (s1 + s2) + s3;
+  st_nt(&fn.xx[q], (s1 + s2) + s3);
   pt = syy1 * ax + ((dt * b) * diff1) / dh; s1 = pt / bx;
   pt = syy2 * ay + ((dt * a) * diff2) / dh; s2 = pt / by;
   pt = syy3 * az + ((dt * b) * diff3) / dh; s3 = pt / bz;
   s[12 * nc] = s1; s[13 * nc] = s2; s[14 * nc] = s3;
-  fn.yy[q] = (s1 + s2) + s3;
+  st_nt(&fn.yy[q], (s1 + s2) + s3);
   pt = szz1 * ax + ((dt * b) * diff1) / dh; s1 = pt / bx;
   pt = szz2 * ay + ((dt * b) * diff2) / dh; s2 = pt / by;
   pt = szz3 * az + ((dt * a) * diff3) / dh; s3 = pt / bz;
   s[15 * nc] = s1; s[16 * nc] = s2; s[17 * nc] = s3;
-  fn.zz[q] = (s1 + s2) + s3;
+  st_nt(&fn.zz[q], (s1 + s2) + s3);
   const bool zero_top = R.ztop && k2 == R.nk;
   if (hxy) {
     const float xmu = (xm + mxy) / 2.0f;
     pt = sxy1 * pamyS + ((dt * xmu) * (uy - u0)) / dh; s1 = pt / papyS;
     pt = sxy2 * pamx + ((dt * xmu) * (v0 - vmx)) / dh; s2 = pt / papx;
     s[18 * nc] = s1; s[19 * nc] = s2;
-    fn.xy[q] = s1 + s2;
+    st_nt(&fn.xy[q], s1 + s2);
   }
   if (hxz) {
     const float xmu = (xm + mxz) / 2.0f;
     pt = sxz1 * pamzS + ((dt * xmu) * (uz - u0)) / dh; s1 = pt / papzS;
     pt = sxz2 * pamx + ((dt * xmu) * (w0 - wmx)) / dh; s2 = pt / papx;
-    fn.xz[q] = s1 + s2;
+    st_nt(&fn.xz[q], s1 + s2);
     if (zero_top && j2 <= R.nzx) { s1 = 0.0f; s2 = 0.0f; }
     s[20 * nc] = s1; s[21 * nc] = s2;
   }
@@ -461,7 +461,7 @@ __device__ inline void pml_str_body(const Dims &g, const Coef &c, const Fields &
     const float xmu = (xm + myz) / 2.0f;
     pt = syz1 * pamzS + ((dt * xmu) * (vz - v0)) / dh; s1 = pt / papzS;
     pt = syz2 * pamyS + ((dt * xmu) * (wy - w0)) / dh; s2 = pt / papyS;
-    fn.yz[q] = s1 + s2;
+    st_nt(&fn.yz[q], s1 + s2);
     if (zero_top && j2 <= R.nzy) { s1 = 0.0f; s2 = 0.0f; }
     s[22 * nc] = s1; s[23 * nc] = s2;
   }

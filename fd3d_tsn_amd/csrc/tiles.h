@@ -21,14 +21,6 @@
 
 namespace fd3d {
 
-// Logical block id of hardware block pid when blocks are dealt round-robin
-// to 8 XCDs: XCD x gets the contiguous logical range [x*N/8, (x+1)*N/8).
-__device__ inline int xcd_remap(int pid, int N) {
-  const int q = N >> 3, r = N & 7;
-  const int xcd = pid & 7, loc = pid >> 3;
-  return xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
-}
-
 template <int TZ>
 struct Tile {
   static constexpr int W = 64 + 4, H = TZ + 4, N = W * H;
@@ -80,7 +72,7 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
     float nxy = 0, nyy = 0, nyz = 0;
     if (adv) { nxy = xy[q + 2 * sy]; nyy = yy[q + 3 * sy]; nyz = yz[q + 2 * sy]; }
     float uo = 0, vo = 0, wo = 0, dq = 1;
-    if (active) { uo = fo.u[q]; vo = fo.v[q]; wo = fo.w[q]; dq = fo.d[q]; }
+    if (active) { uo = ld_nt(&fo.u[q]); vo = ld_nt(&fo.v[q]); wo = ld_nt(&fo.w[q]); dq = ld_nt(&fo.d[q]); }
     L[0][a0] = xx0; L[1][a0] = xy0; L[2][a0] = xz0; L[3][a0] = yz0; L[4][a0] = zz0;
     for (int h = t; h < NHTOT; h += 64 * TZ) {
       int fld, e = h;
@@ -104,15 +96,15 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
       const float *Lxx = L[0], *Lxy = L[1], *Lxz = L[2], *Lyz = L[3], *Lzz = L[4];
       const int W = T::W;
       const float b = c.dth / dq;
-      fn.u[q] = uo + b * (c1 * (xx0 - Lxx[a0 - 1]) + c2 * (Lxx[a0 + 1] - Lxx[a0 - 2]) +
+      st_nt(&fn.u[q], uo + b * (c1 * (xx0 - Lxx[a0 - 1]) + c2 * (Lxx[a0 + 1] - Lxx[a0 - 2]) +
                              c1 * (xy0 - xym1) + c2 * (xyp1 - xym2) +
-                             c1 * (xz0 - Lxz[a0 - W]) + c2 * (Lxz[a0 + W] - Lxz[a0 - 2 * W]));
-      fn.v[q] = vo + b * (c1 * (Lxy[a0 + 1] - xy0) + c2 * (Lxy[a0 + 2] - Lxy[a0 - 1]) +
+                             c1 * (xz0 - Lxz[a0 - W]) + c2 * (Lxz[a0 + W] - Lxz[a0 - 2 * W])));
+      st_nt(&fn.v[q], vo + b * (c1 * (Lxy[a0 + 1] - xy0) + c2 * (Lxy[a0 + 2] - Lxy[a0 - 1]) +
                              c1 * (yyp1 - yy0) + c2 * (yyp2 - yym1) +
-                             c1 * (yz0 - Lyz[a0 - W]) + c2 * (Lyz[a0 + W] - Lyz[a0 - 2 * W]));
-      fn.w[q] = wo + b * (c1 * (Lxz[a0 + 1] - xz0) + c2 * (Lxz[a0 + 2] - Lxz[a0 - 1]) +
+                             c1 * (yz0 - Lyz[a0 - W]) + c2 * (Lyz[a0 + W] - Lyz[a0 - 2 * W])));
+      st_nt(&fn.w[q], wo + b * (c1 * (Lxz[a0 + 1] - xz0) + c2 * (Lxz[a0 + 2] - Lxz[a0 - 1]) +
                              c1 * (yz0 - yzm1) + c2 * (yzp1 - yzm2) +
-                             c1 * (Lzz[a0 + W] - zz0) + c2 * (Lzz[a0 + 2 * W] - Lzz[a0 - W]));
+                             c1 * (Lzz[a0 + W] - zz0) + c2 * (Lzz[a0 + 2 * W] - Lzz[a0 - W])));
     }
     if (adv) {
       xym2 = xym1; xym1 = xy0; xy0 = xyp1; xyp1 = nxy;
