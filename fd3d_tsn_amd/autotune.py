@@ -1,7 +1,8 @@
 """Autotuner of the launch shapes of the 3D sweeps, per problem size class.
 
-The interior sweeps (tiles.h) take a tile height tz (4, 8, 16 lanes along
-z, 64 along x) and a number ny of rows each block marches along y; the
+The interior sweeps (tiles.h) take a tile of 64 x tz (tz = 4, 8, 16) or
+128 x tz (tz = 4, 8) cells of the x-z plane and a number ny of rows each
+block marches along y; the
 PML sweeps run one thread per cell or march z in register windows.  The
 best shape depends on the grid: narrow grids waste lanes on tall tiles,
 short y extents leave too few blocks when ny is large.  Results never
@@ -58,13 +59,29 @@ def _save(db: Dict):
     os.replace(tmp, CACHE)
 
 
-def candidates(case: C.Case) -> List[Tuple[int, int]]:
+WIDE_TZS = (4, 8)
+
+
+def key(wide: bool, tz: int, ny: int) -> str:
+    """Table key of a shape: "16x32" is 64 x 16 tiles marching 32 rows,
+    "w8x32" is 128 x 8 tiles marching 32 rows."""
+    return f"{'w' if wide else ''}{tz}x{ny}"
+
+
+def parse(k: str) -> Tuple[bool, int, int]:
+    wide = k.startswith("w")
+    tz, ny = (int(v) for v in k.lstrip("w").split("x"))
+    return wide, tz, ny
+
+
+def candidates(case: C.Case) -> List[Tuple[bool, int, int]]:
     nyi = case.nyt - 2 - case.nabc
     out = []
-    for tz in TZS:
-        for ny in NYS:
-            if ny <= max(8, 2 * nyi):
-                out.append((tz, ny))
+    for wide, tzs in ((False, TZS), (True, WIDE_TZS)):
+        for tz in tzs:
+            for ny in NYS:
+                if ny <= max(8, 2 * nyi):
+                    out.append((wide, tz, ny))
     return out
 
 
@@ -75,18 +92,17 @@ def tune(solver: "capi.Solver", steps: int = 3, warmup: int = 1, save: bool = Tr
     case = solver.case
     before = solver.get_tuning()
     table = {"vel": {}, "str": {}, "pml": {}}
-    for tz, ny in candidates(case):
-        for pml_mode in ((1, 3) if (tz, ny) == (16, 32) else (1,)):
-            solver.set_tuning(pml_mode, tz, ny, tz, ny)
+    for wide, tz, ny in candidates(case):
+        for pml_mode in ((1, 3) if (wide, tz, ny) == (False, 16, 32) else (1,)):
+            solver.set_tuning(pml_mode | (24 if wide else 0), tz, ny, tz, ny)
             solver.step(warmup, outputs=False)
             solver.set_profiling(True)
             solver.step(steps, outputs=False)
             kt = solver.kernel_times()
             solver.set_profiling(False)
-            key = f"{tz}x{ny}"
             if pml_mode == 1:
-                table["vel"][key] = kt["vel_interior"][0] / kt["vel_interior"][1]
-                table["str"][key] = kt["str_interior"][0] / kt["str_interior"][1]
+                table["vel"][key(wide, tz, ny)] = kt["vel_interior"][0] / kt["vel_interior"][1]
+                table["str"][key(wide, tz, ny)] = kt["str_interior"][0] / kt["str_interior"][1]
             if "pml_vel" in kt:
                 pm = (kt["pml_vel"][0] / kt["pml_vel"][1]) + (kt["pml_str"][0] / kt["pml_str"][1])
                 table["pml"]["zmarch" if pml_mode & 2 else "flat"] = pm
@@ -98,14 +114,16 @@ def tune(solver: "capi.Solver", steps: int = 3, warmup: int = 1, save: bool = Tr
     return table
 
 
-def _best(tab: Dict[str, float], tz: Optional[int] = None) -> Tuple[int, int, float]:
+def _best(tab: Dict[str, float], tz: Optional[int] = None) -> Optional[Tuple[bool, int, int, float]]:
+    """Fastest (wide, tz, ny, ms) of a table; with tz given, among the
+    64-wide shapes of that height only (the batch engine's tiles)."""
     best = None
     for k, ms in tab.items():
-        t, n = (int(v) for v in k.split("x"))
-        if tz is not None and t != tz:
+        w, t, n = parse(k)
+        if tz is not None and (w or t != tz):
             continue
-        if best is None or ms < best[2]:
-            best = (t, n, ms)
+        if best is None or ms < best[3]:
+            best = (w, t, n, ms)
     return best
 
 
@@ -118,11 +136,11 @@ def choose(case: C.Case, tz: Optional[int] = None) -> Optional[Tuple[int, int, i
     e = lookup(case)
     if not e or not e["vel"]:
         return None
-    tv, nv, _ = _best(e["vel"], tz) or (None, None, None)
-    ts, ns, _ = _best(e["str"], tz) or (None, None, None)
+    wv, tv, nv, _ = _best(e["vel"], tz) or (None, None, None, None)
+    ws, ts, ns, _ = _best(e["str"], tz) or (None, None, None, None)
     if tv is None or ts is None:
         return None
-    mode = 1
+    mode = 1 | (8 if wv else 0) | (16 if ws else 0)
     if e.get("pml") and e["pml"].get("zmarch", 1e30) < e["pml"].get("flat", 1e30):
         mode = 3
     return (mode, tv, nv, ts, ns)
@@ -137,20 +155,23 @@ def apply(solver: "capi.Solver", tz: Optional[int] = None) -> bool:
 
 
 def batch_plan(cases: Iterable[C.Case]) -> Tuple[int, List[Optional[Tuple[int, int, int, int, int]]]]:
-    """Common tile height for a batch and each member's shape at that height."""
+    """Common tile height for a batch and each member's shape at that height:
+    the height with the least summed sweep time over the measured members,
+    among the heights measured for every one of them (16 when none is)."""
     cases = list(cases)
+    tables = [e for e in (lookup(c) for c in cases) if e]
     totals = {}
     for tz in TZS:
         tot = 0.0
-        for c in cases:
-            e = lookup(c)
-            if not e:
-                continue
+        for e in tables:
             bv, bs = _best(e["vel"], tz), _best(e["str"], tz)
-            if bv and bs:
-                tot += bv[2] + bs[2]
-        totals[tz] = tot
-    tz = min(TZS, key=lambda t: totals[t]) if any(totals.values()) else 16
+            if not (bv and bs):
+                tot = None
+                break
+            tot += bv[3] + bs[3]
+        if tot is not None and tables:
+            totals[tz] = tot
+    tz = min(totals, key=totals.get) if totals else 16
     return tz, [choose(c, tz) for c in cases]
 
 
@@ -178,7 +199,7 @@ def main():
         with capi.Solver(c.replace(ntfd=8, stations=[])) as s:
             tab = tune(s)
         bv, bs = _best(tab["vel"]), _best(tab["str"])
-        print(f"[autotune] {k}: vel {bv[0]}x{bv[1]} {bv[2]:.3f} ms, str {bs[0]}x{bs[1]} {bs[2]:.3f} ms "
+        print(f"[autotune] {k}: vel {key(*bv[:3])} {bv[3]:.3f} ms, str {key(*bs[:3])} {bs[3]:.3f} ms "
               f"({time.perf_counter()-t0:.1f} s)", flush=True)
 
 

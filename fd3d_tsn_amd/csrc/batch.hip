@@ -86,7 +86,7 @@ __device__ __forceinline__ void run_item(const BatchDomain &D, int stage, int b,
   const Fields &fo = FP.fo, &fn = FP.fn;
   switch (stage) {
     case K_VEL_INTERIOR:
-      vel_tiled_body<TZ>(P.g, P.c, fo, fn, lds, b % D.gx, (b / D.gx) % D.gz, b / (D.gx * D.gz), lx, ly, D.nyv);
+      vel_tiled_body<64, TZ>(P.g, P.c, fo, fn, lds, b % D.gx, (b / D.gx) % D.gz, b / (D.gx * D.gz), lx, ly, D.nyv);
       break;
     case K_VEL_FAULT:
       vel_fault_body<TZ, FVW>(P.g, P.c, fo, fn, P.fl, P.fv, P.ctl, b % D.fpx, b / D.fpx, lx, ly);
@@ -103,7 +103,7 @@ __device__ __forceinline__ void run_item(const BatchDomain &D, int stage, int b,
       fuvw_w_body<TZ>(P.g, fo, fn, b % D.fpx, b / D.fpx, lx, ly);
       break;
     case K_STR_INTERIOR:
-      str_tiled_body<TZ>(P.g, P.c, fo, fn, lds, b % D.gx, (b / D.gx) % D.gz, b / (D.gx * D.gz), lx, ly, D.nys);
+      str_tiled_body<64, TZ>(P.g, P.c, fo, fn, lds, b % D.gx, (b / D.gx) % D.gz, b / (D.gx * D.gz), lx, ly, D.nys);
       break;
     case K_STR_FAULT:
       str_fault_body<TZ>(P.g, P.c, fo, fn, P.fl, b % D.fpx, b / D.fpx, lx, ly);
@@ -160,7 +160,7 @@ __device__ inline int find_seg(const BatchSeg *seg, int lo, int hi, int item) {
 
 template <int TZ, bool FVW>
 __global__ void __launch_bounds__(64 * TZ) k_batch_persistent(BatchArgs A) {
-  __shared__ float lds[Tile<TZ>::LDS_FLOATS];
+  __shared__ float lds[Tile<64, TZ>::LDS_FLOATS];
   __shared__ int s_item, s_seg;
   const int t = threadIdx.y * 64 + threadIdx.x;
   for (;;) {
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(64 * TZ) k_batch_persistent(BatchArgs A) {
 // instantiation holds (and allocates registers for) one stage body only.
 template <int TZ, bool FVW, int STAGE>
 __global__ void __launch_bounds__(64 * TZ) k_batch_stage(BatchArgs A, int seg_lo, int seg_hi, int item0) {
-  __shared__ float lds[STAGE == K_VEL_INTERIOR || STAGE == K_STR_INTERIOR ? Tile<TZ>::LDS_FLOATS
+  __shared__ float lds[STAGE == K_VEL_INTERIOR || STAGE == K_STR_INTERIOR ? Tile<64, TZ>::LDS_FLOATS
                        : STAGE == K_RATES                                  ? 2 * 64 * TZ
                                                                            : 1];
   const int item = item0 + (int)blockIdx.x;

@@ -127,7 +127,8 @@ struct Outputs {
 // the autotuner.  mode bit 0: tiled interior sweeps (interior.hip) instead
 // of one thread per cell; bit 1: z-march PML sweeps (pml.hip) instead of
 // one thread per cell (kernels.hip); bit 2: the fused interior sweep
-// (fused.hip, full space only; rows per block from ny_vel).
+// (fused.hip, full space only; rows per block from ny_vel); bits 3 and 4:
+// velocity and stress tiles 128 cells wide along x instead of 64.
 struct Tuning {
   int mode;
   int tz_vel, ny_vel;   // velocity: x-z tile height (4, 8, 16), rows per block along y

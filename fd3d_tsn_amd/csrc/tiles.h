@@ -2,14 +2,17 @@
 // fd3d_deriv.f90:24-193) and stress (dstres, :513-622) updates over
 // i in [nabc+1, nxt-nabc], j in [nabc+1, nyt-2], k in [nabc+1, nzt-nfs].
 //
-// Layout of the work: a block owns a 64 x TZ tile of the x-z plane (one
-// wavefront along x per z row) and marches along y over NY rows.
+// Layout of the work: a block owns a TX x TZ tile of the x-z plane (TX = 64
+// or 128 lanes along x, TZ rows along z) and marches along y over NY rows.
 //  * y neighbours come from per-thread register windows (each value is
 //    loaded once per block);
 //  * x and z neighbours come from the x-z plane of the current row staged
-//    in LDS, double-buffered so one barrier per row suffices; halo cells
-//    (2 columns / 2 rows on each side) are loaded by the block, centre
-//    cells are written from the registers that already hold them.
+//    in LDS, double-buffered so one barrier per row suffices; centre cells
+//    are written from the registers that already hold them, and the block
+//    loads exactly the halo cells its stencils read (three columns or three
+//    rows per field, on the sides that field's differences reach), so no
+//    corner cell and no unused row of a neighbour tile is fetched.  Wider
+//    tiles fetch fewer neighbour cache lines per cell for the x halos.
 // The bodies take the LDS buffer (Tile<TZ>::LDS_FLOATS floats) and the
 // tile coordinates, so that single-problem launches (interior.hip) and the
 // multi-domain batch kernels (batch.hip) share them.  A caller reusing the
@@ -21,9 +24,10 @@
 
 namespace fd3d {
 
-template <int TZ>
+template <int TX, int TZ>
 struct Tile {
-  static constexpr int W = 64 + 4, H = TZ + 4, N = W * H;
+  static_assert((TX == 64 || TX == 128) && TX * TZ <= 1024, "tile shape");
+  static constexpr int W = TX + 4, H = TZ + 4, N = W * H;
   static constexpr int LDS_FLOATS = 2 * 5 * N;   // double-buffered planes, up to 5 fields
   __device__ static int at(int x, int z) { return (z + 2) * W + (x + 2); }
 };
@@ -33,13 +37,13 @@ struct Tile {
 // yz (j-2..j+1).  LDS x-z plane at row j: xx (x halo), xy (x), xz (x, z),
 // yz (z), zz (z).
 // ---------------------------------------------------------------------------
-template <int TZ>
+template <int TX, int TZ>
 __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, float *lds,
                                       int bx, int bz,
                                       int by, int tx, int tz, int NY) {
-  using T = Tile<TZ>;
-  const int t = tz * 64 + tx;
-  const int i0 = g.nabc + 1 + bx * 64, kk0 = g.nabc + 1 + bz * TZ;
+  using T = Tile<TX, TZ>;
+  const int t = tz * TX + tx;
+  const int i0 = g.nabc + 1 + bx * TX, kk0 = g.nabc + 1 + bz * TZ;
   const int i = i0 + tx, k = kk0 + tz;
   const int j0 = g.nabc + 1 + by * NY;
   const int je = min(j0 + NY - 1, g.nyt - 2);
@@ -56,8 +60,8 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
     yym1 = yy[q - sy]; yy0 = yy[q]; yyp1 = yy[q + sy]; yyp2 = yy[q + 2 * sy];
     yzm2 = yz[q - 2 * sy]; yzm1 = yz[q - sy]; yz0 = yz[q]; yzp1 = yz[q + sy];
   }
-  constexpr int NX = 4 * TZ, NZH = 4 * 68;
-  constexpr int NHTOT = NX + NX + (NX + NZH) + NZH + NZH;
+  // halo cells per field: x halos 3 columns x TZ rows, z halos 3 rows x TX
+  constexpr int NXH = 3 * TZ, NZH = 3 * TX, NHTOT = 3 * NXH + 3 * NZH;
   int buf = 0;
   for (int j = j0; j <= je; ++j) {
     const int qj = I3(g, 0, j, 1);  // element (i,j,k) at qj + i + (k-1)*sz
@@ -74,17 +78,21 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
     float uo = 0, vo = 0, wo = 0, dq = 1;
     if (active) { uo = ld_nt(&fo.u[q]); vo = ld_nt(&fo.v[q]); wo = ld_nt(&fo.w[q]); dq = ld_nt(&fo.d[q]); }
     L[0][a0] = xx0; L[1][a0] = xy0; L[2][a0] = xz0; L[3][a0] = yz0; L[4][a0] = zz0;
-    for (int h = t; h < NHTOT; h += 64 * TZ) {
-      int fld, e = h;
-      bool xh;
-      if (e < NX) { fld = 0; xh = true; }
-      else if ((e -= NX) < NX) { fld = 1; xh = true; }
-      else if ((e -= NX) < NX + NZH) { fld = 2; xh = e < NX; if (!xh) e -= NX; }
-      else if ((e -= NX + NZH) < NZH) { fld = 3; xh = false; }
-      else { e -= NZH; fld = 4; xh = false; }
-      int hx, hz;
-      if (xh) { hz = e >> 2; const int c4 = e & 3; hx = c4 < 2 ? c4 - 2 : 62 + c4; }
-      else { const int r = e / 68, cc = e % 68; hz = r < 2 ? r - 2 : TZ + r - 2; hx = cc - 2; }
+    for (int h = t; h < NHTOT; h += TX * TZ) {
+      int fld, hx, hz;
+      if (h < 3 * NXH) {
+        // x halos of xx (columns -2, -1, TX), xy and xz (-1, TX, TX+1)
+        const int f = h / NXH, e = h - f * NXH, r = e / 3, ci = e - 3 * r;
+        fld = f;
+        hz = r;
+        hx = f == 0 ? (ci == 2 ? TX : ci - 2) : (ci == 0 ? -1 : TX + ci - 1);
+      } else {
+        // z halos of xz, yz (rows -2, -1, TZ) and zz (-1, TZ, TZ+1)
+        const int e0 = h - 3 * NXH, f = e0 / NZH, e = e0 - f * NZH, r = e / TX;
+        fld = f + 2;
+        hx = e - r * TX;
+        hz = f < 2 ? (r == 2 ? TZ : r - 2) : (r == 0 ? -1 : TZ + r - 1);
+      }
       const int gi = i0 + hx, gk = kk0 + hz;
       const float *s = fld == 0 ? xx : fld == 1 ? xy : fld == 2 ? xz : fld == 3 ? yz : zz;
       float val = 0.0f;
@@ -119,13 +127,13 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
 // Stress.  Register windows along y: v (j-2..j+1), u (j-1..j+2),
 // w (j-1..j+2), mu (j, j+1).  LDS x-z plane at row j: u, v, w, mu.
 // ---------------------------------------------------------------------------
-template <int TZ>
+template <int TX, int TZ>
 __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, float *lds,
                                       int bx, int bz,
                                       int by, int tx, int tz, int NY) {
-  using T = Tile<TZ>;
-  const int t = tz * 64 + tx;
-  const int i0 = g.nabc + 1 + bx * 64, kk0 = g.nabc + 1 + bz * TZ;
+  using T = Tile<TX, TZ>;
+  const int t = tz * TX + tx;
+  const int i0 = g.nabc + 1 + bx * TX, kk0 = g.nabc + 1 + bz * TZ;
   const int i = i0 + tx, k = kk0 + tz;
   const int j0 = g.nabc + 1 + by * NY;
   const int je = min(j0 + NY - 1, g.nyt - 2);
@@ -143,7 +151,9 @@ __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields
     wm1 = w[q - sy]; w0 = w[q]; wp1 = w[q + sy]; wp2 = w[q + 2 * sy];
     m0 = mu[q]; mp1 = mu[q + sy];
   }
-  constexpr int NH = 4 * TZ + 4 * 68;
+  // halo cells per field: x halos 3 columns x TZ rows, z halos 3 rows x TX
+  // (mu: one row)
+  constexpr int NXH = 3 * TZ, NZH = 3 * TX, NHTOT = 3 * NXH + 3 * NZH + TX;
   int buf = 0;
   for (int j = j0; j <= je; ++j) {
     const int qj = I3(g, 0, j, 1);
@@ -161,11 +171,26 @@ __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields
       oxx = fo.xx[q]; oyy = fo.yy[q]; ozz = fo.zz[q]; oxy = fo.xy[q]; oxz = fo.xz[q]; oyz = fo.yz[q];
       xl = fo.lam[q]; mdiag = mu[q + 1 + sy + sz];
     }
-    for (int h = t; h < 4 * NH; h += 64 * TZ) {
-      const int fld = h / NH, e = h - fld * NH;
-      int hx, hz;
-      if (e < 4 * TZ) { hz = e >> 2; const int c4 = e & 3; hx = c4 < 2 ? c4 - 2 : 62 + c4; }
-      else { const int r = (e - 4 * TZ) / 68, cc = (e - 4 * TZ) % 68; hz = r < 2 ? r - 2 : TZ + r - 2; hx = cc - 2; }
+    for (int h = t; h < NHTOT; h += TX * TZ) {
+      int fld, hx, hz;
+      if (h < 3 * NXH) {
+        // x halos of u (columns -1, TX, TX+1), v and w (-2, -1, TX)
+        const int f = h / NXH, e = h - f * NXH, r = e / 3, ci = e - 3 * r;
+        fld = f;
+        hz = r;
+        hx = f == 0 ? (ci == 0 ? -1 : TX + ci - 1) : (ci == 2 ? TX : ci - 2);
+      } else if (h < 3 * NXH + 3 * NZH) {
+        // z halos of u, v (rows -1, TZ, TZ+1) and w (-2, -1, TZ)
+        const int e0 = h - 3 * NXH, f = e0 / NZH, e = e0 - f * NZH, r = e / TX;
+        fld = f;
+        hx = e - r * TX;
+        hz = f < 2 ? (r == 0 ? -1 : TZ + r - 1) : (r == 2 ? TZ : r - 2);
+      } else {
+        // mu, row TZ
+        fld = 3;
+        hx = h - 3 * NXH - 3 * NZH;
+        hz = TZ;
+      }
       const int gi = i0 + hx, gk = kk0 + hz;
       const float *s = fld == 0 ? u : fld == 1 ? v : fld == 2 ? w : mu;
       float val = 0.0f;

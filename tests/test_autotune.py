@@ -1,0 +1,48 @@
+"""Shape selection of the autotuner from a cached table (no GPU)."""
+import json
+
+import pytest
+
+from fd3d_tsn_amd import autotune as A
+from fd3d_tsn_amd import cases as C
+
+
+def test_key_round_trip():
+    for shape in ((False, 16, 32), (True, 8, 64), (True, 4, 1)):
+        assert A.parse(A.key(*shape)) == shape
+
+
+def test_candidates_cover_wide_tiles():
+    cs = A.candidates(C.c2_large_sw())
+    assert (True, 8, 32) in cs and (False, 16, 32) in cs
+    assert all(tz in A.WIDE_TZS for w, tz, _ in cs if w)
+
+
+@pytest.fixture
+def cache(tmp_path, monkeypatch):
+    path = tmp_path / "cache.json"
+    monkeypatch.setattr(A, "CACHE", str(path))
+    return path
+
+
+def test_choose_sets_width_bits(cache):
+    case = C.c2_large_sw()
+    tab = {"vel": {"16x32": 3.0, "w8x32": 2.5, "8x64": 3.2},
+           "str": {"16x32": 4.0, "w8x16": 4.1, "4x8": 5.0},
+           "pml": {"flat": 3.0, "zmarch": 3.5}}
+    json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
+    assert A.choose(case) == (1 | 8, 8, 32, 16, 32)
+    # a fixed tile height (the batch engine's) uses 64-wide shapes only
+    assert A.choose(case, tz=16) == (1, 16, 32, 16, 32)
+    assert A.choose(case, tz=8) is None   # no 64 x 8 stress shape measured
+
+
+def test_batch_plan_minimises_summed_time(cache):
+    a, b = C.c2_large_sw(), C.c3_fvw()
+    json.dump({"entries": {
+        A.size_class(a): {"vel": {"16x32": 3.0, "8x32": 3.5}, "str": {"16x32": 4.0, "8x32": 4.2}, "pml": {}},
+        A.size_class(b): {"vel": {"16x32": 1.0, "8x32": 0.2}, "str": {"16x32": 1.0, "8x32": 0.3}, "pml": {}},
+    }}, open(cache, "w"))
+    tz, plan = A.batch_plan([a, b])
+    assert tz == 8                      # 3.5 + 4.2 + 0.2 + 0.3 < 3 + 4 + 1 + 1
+    assert plan[0] == (1, 8, 32, 8, 32) and plan[1] == (1, 8, 32, 8, 32)

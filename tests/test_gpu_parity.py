@@ -140,7 +140,8 @@ def test_fvw_against_mixed_and_fp64(name, kw):
     assert (rt_g < 1e4).sum() > 0, "no node ruptured"
 
 
-TUNINGS = [(0, 16, 64, 16, 64), (1, 4, 1, 4, 3), (3, 8, 5, 16, 2), (2, 16, 64, 8, 7), (3, 4, 1000, 4, 1000)]
+TUNINGS = [(0, 16, 64, 16, 64), (1, 4, 1, 4, 3), (3, 8, 5, 16, 2), (2, 16, 64, 8, 7), (3, 4, 1000, 4, 1000),
+           (9, 8, 5, 4, 3), (25, 4, 2, 8, 7), (19, 16, 3, 8, 1000)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS, ids=[f"m{t[0]}_{t[1]}x{t[2]}_{t[3]}x{t[4]}" for t in TUNINGS])
@@ -161,6 +162,37 @@ def test_launch_shapes_bit_exact(tuning):
         assert np.array_equal(rg[n].astype(np.float64), ro[n]), n
     for n in capi.FIELDS:
         assert np.array_equal(fields[n].astype(np.float64), orc.field(n)), n
+
+
+WIDE = [(25, 8, 4, 8, 3), (25, 4, 1, 4, 64), (9, 8, 2, 16, 5), (17, 4, 3, 8, 1)]
+
+
+@pytest.mark.parametrize("tuning", WIDE, ids=[f"m{t[0]}_{t[1]}x{t[2]}_{t[3]}x{t[4]}" for t in WIDE])
+def test_wide_tiles_bit_exact(tuning):
+    """128-wide interior tiles on a grid of two full tiles and a partial
+    one along x, with a free surface: the oracle's wavefield bit for bit."""
+    case = C.tiny(friction="sw", fspace=False, nabc=4, n=(268, 11, 23), ntfd=12, seed=37)
+    say(f"[parity] wide tiles {tuning} grid {case.nxt}x{case.nyt}x{case.nzt}")
+    orc = O.Oracle(case, O.F32)
+    ro = orc.step(case.ntfd)
+    with capi.Solver(case) as s:
+        s.set_tuning(*tuning)
+        assert s.get_tuning() == tuning
+        rg = s.step(case.ntfd)
+        fields = {n: s.field(n) for n in capi.FIELDS}
+    for n in OUTS:
+        assert np.array_equal(rg[n].astype(np.float64), ro[n]), n
+    for n in capi.FIELDS:
+        assert np.array_equal(fields[n].astype(np.float64), orc.field(n)), n
+
+
+def test_wide_tiles_reject_bad_shapes():
+    case = C.tiny(friction="sw", fspace=True, seed=38)
+    with capi.Solver(case) as s:
+        for bad in ((8, 8, 4, 8, 4), (9, 16, 4, 8, 4), (17, 8, 4, 16, 4), (32, 8, 4, 8, 4)):
+            with pytest.raises(capi.FD3DError) as e:
+                s.set_tuning(*bad)
+            assert e.value.code == -1, bad
 
 
 FUSED = [("fs_ny3", dict(nabc=5, n=(70, 17, 21), seed=33), 3),
