@@ -26,13 +26,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per cell of each sweep (FP32 words x 4)
+# Algorithmic bytes per cell of each sweep (FP32 words x 4): the words each
+# sweep must move.  The medium is depth-layered (one lam, mu, rho per z
+# plane, as the reference builds it), so no sweep streams medium arrays.
 BYTES_PER_CELL = {
-    "vel_interior": 13 * 4,   # reads 6 stresses + rho + u, v, w; writes u, v, w
-    "str_interior": 17 * 4,   # reads u, v, w, lam, mu + 6 stresses; writes 6 stresses
-    "pml_vel": 28 * 4,        # 9 splits read+write, 6 stresses + rho read, 3 velocities written
-    "pml_str": 41 * 4,        # 15 splits read+write, 3 velocities + lam + mu read, 6 stresses written
+    "vel_interior": 12 * 4,   # reads 6 stresses + u, v, w; writes u, v, w
+    "str_interior": 15 * 4,   # reads u, v, w + 6 stresses; writes 6 stresses
+    "pml_vel": 27 * 4,        # 9 splits read+write, 6 stresses read, 3 velocities written
+    "pml_str": 39 * 4,        # 15 splits read+write, 3 velocities read, 6 stresses written
 }
+# The same per step: 108 B per bulk cell, 264 B per PML cell.  The survey's
+# convention counts the medium as three streamed 3D arrays (120 B / 276 B);
+# the JSON line reports the HBM fraction on both.
+MOVED_BULK, MOVED_PML = 108.0, 264.0
 
 
 def parse():
@@ -143,7 +149,7 @@ def bench_batched(a):
         s.step(a.steps, outputs=False)
         s.sync()
         plain.append((time.perf_counter() - t1) / a.steps)
-    step_b = [s.step_bytes()[0] for s in solvers]
+    step_b = [MOVED_BULK * nb + MOVED_PML * npml for _, nb, npml in (s.step_bytes() for s in solvers)]
     for s, p in zip(solvers, plan):
         s.init()
         if p is not None:
@@ -273,6 +279,7 @@ def main():
     kt = s.kernel_times()
     s.set_profiling(False)
     step_bytes, nbulk, npml = s.step_bytes()
+    moved = MOVED_BULK * nbulk + MOVED_PML * npml
     tuning = s.get_tuning()
     s.close()
 
@@ -305,9 +312,13 @@ def main():
                    "dt": case.dt, "nabc": case.nabc, "friction": case.friction,
                    "fspace": case.fspace, "cells": case.cells, "tuning": list(tuning),
                    "autotuned": tuned},
-        "hbm_step": {"alg_bytes_per_step": step_bytes, "bulk_cells": nbulk, "pml_cells": npml,
-                     "achieved_GBs": step_bytes / (ms_per_step * 1e-3) / 1e9,
-                     "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "hbm_step": {"alg_bytes_per_step": moved, "bulk_cells": nbulk, "pml_cells": npml,
+                     "achieved_GBs": moved / (ms_per_step * 1e-3) / 1e9,
+                     "frac": moved / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "convention_bytes_per_step": step_bytes,
+                     "convention_frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "note": "alg bytes: 108 B/bulk cell, 264 B/PML cell (layered medium, no medium "
+                             "arrays streamed); convention: 120/276 B with the medium as three 3D arrays"},
         "roofline": roof,
         "kernels": kernels,
     }

@@ -194,22 +194,26 @@ class Solver:
             self.h = None
             raise FD3DError(rc, "fd3d_create failed")
         t2 = clock()
-        if medium is None:
-            medium = C.material(case, np.float32)
-        lam, mu, d = (_f32(x) for x in medium)
-        self._chk(self.L.fd3d_upload_medium(self.h, _fp(lam), _fp(mu), _fp(d)))
-        t3 = clock()
-        if fvw:
-            arrs = [_f64(fault[n]) for n in C.FVW_FIELDS]
-            self._chk(self.L.fd3d_upload_fault_fvw(self.h, *[_dp(a) for a in arrs]))
-        else:
-            if fault is None:
-                fault = C.fault_sw(case, np.float32)
-            arrs = [_f32(fault[n]) for n in C.SW_FIELDS]
-            self._chk(self.L.fd3d_upload_fault_sw(self.h, *[_fp(a) for a in arrs]))
-        t4 = clock()
-        if init:
-            self.init()
+        try:
+            if medium is None:
+                medium = C.material(case, np.float32)
+            lam, mu, d = (_f32(x) for x in medium)
+            self._chk(self.L.fd3d_upload_medium(self.h, _fp(lam), _fp(mu), _fp(d)))
+            t3 = clock()
+            if fvw:
+                arrs = [_f64(fault[n]) for n in C.FVW_FIELDS]
+                self._chk(self.L.fd3d_upload_fault_fvw(self.h, *[_dp(a) for a in arrs]))
+            else:
+                if fault is None:
+                    fault = C.fault_sw(case, np.float32)
+                arrs = [_f32(fault[n]) for n in C.SW_FIELDS]
+                self._chk(self.L.fd3d_upload_fault_sw(self.h, *[_fp(a) for a in arrs]))
+            t4 = clock()
+            if init:
+                self.init()
+        except Exception:
+            self.close()   # release the device memory of a half-built problem
+            raise
         # host seconds of each set-up phase
         self.setup_times = {"inputs": t1 - t0, "create": t2 - t1, "medium": t3 - t2, "fault": t4 - t3,
                             "init": clock() - t4}

@@ -226,7 +226,8 @@ size_t footprint(const fd3d_config &c) {
   if (build_regions(c, g, pm, nsplit, nprof) != 0) return 0;
   const size_t n2 = (size_t)c.nxt * c.nzt;
   const int ring = c.ring_steps > 0 ? c.ring_steps : 64;
-  size_t b = 21 * (size_t)g.n3 * 4;                 // 3D fields: two wavefield sets + medium
+  size_t b = 18 * (size_t)g.n3 * 4;                 // 3D fields: two wavefield sets
+  b += 3 * (size_t)(c.nzt + 2) * 4;                 // medium: one value per layer (z plane)
   b += (size_t)nsplit * 4 + (size_t)nprof * 4;      // PML
   b += n2 * 4 * 33 + n2 * 8 * 8;                    // fault plane
   b += (size_t)ring * FD3D_NOUT * g.nxT * g.nzT * 4 + (size_t)ring * c.nstations * 3 * 4;
@@ -360,13 +361,19 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
   };
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(FD3D_E_HIP);
   const size_t n3 = (size_t)p.g.n3, n2 = (size_t)cfg->nxt * cfg->nzt;
-  // two wavefield sets (the step reads one and writes the other), one medium
+  // two wavefield sets (the step reads one and writes the other)
   Fields &A = p.f[0], &B = p.f[1];
   float **f3[] = {&A.u, &A.v, &A.w, &A.xx, &A.yy, &A.zz, &A.xy, &A.yz, &A.xz,
-                  &B.u, &B.v, &B.w, &B.xx, &B.yy, &B.zz, &B.xy, &B.yz, &B.xz,
-                  &A.lam, &A.mu, &A.d};
+                  &B.u, &B.v, &B.w, &B.xx, &B.yy, &B.zz, &B.xy, &B.yz, &B.xz};
   for (int q = 0; q < (int)(sizeof(f3) / sizeof(f3[0])); ++q)
     if ((rc = dalloc(h, f3[q], n3, field_skew(q)))) return bail(rc);
+  // the depth-layered medium: lam, mu, rho of plane k at index k (1..nzt;
+  // entries 0 and nzt+1 stay zero)
+  const size_t nlay = (size_t)cfg->nzt + 2;
+  for (float **pp : {&A.lam, &A.mu, &A.d}) {
+    if ((rc = dalloc(h, pp, nlay))) return bail(rc);
+    if (hipMemsetAsync(*pp, 0, nlay * sizeof(float), h->stream) != hipSuccess) return bail(FD3D_E_HIP);
+  }
   B.lam = A.lam; B.mu = A.mu; B.d = A.d;
   Fault &fl = p.fl;
   float **f2[] = {&fl.tx, &fl.tz, &fl.v1t, &fl.avdx, &fl.avdz, &fl.RFx, &fl.RFz, &fl.au1, &fl.av1,
@@ -435,24 +442,37 @@ static int download3(fd3d_handle *h, float *dst, const float *src) {
   return 0;
 }
 
+// The reference's medium is depth-layered: readcrustalmodel fills whole
+// x-y planes, lam1(1:nxt,1:nyt,k) = ... (fd3d_init.f90:240-265).  The
+// device keeps one value per plane, so the sweeps read no medium arrays.
 int fd3d_upload_medium(fd3d_handle *h, const float *lam1, const float *mu1, const float *d1) {
   if (!h || !lam1 || !mu1 || !d1) return fail(h, FD3D_E_INVALID, "null argument");
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  const int nzt = h->cfg.nzt;
+  const size_t plane = (size_t)h->cfg.nxt * h->cfg.nyt;
+  std::vector<float> lay(3 * ((size_t)nzt + 2), 0.0f);
+  float *L = lay.data(), *M = L + nzt + 2, *D = M + nzt + 2;
   // CFL check on the largest P velocity (fd3d_init.f90:274-278)
-  const size_t n = (size_t)h->cfg.nxt * h->cfg.nyt * h->cfg.nzt;
   float vmax = 0.0f;
-  for (size_t q = 0; q < n; ++q) {
-    if (!(d1[q] > 0.0f)) return fail(h, FD3D_E_INVALID, "density must be positive");
-    const float vp = std::sqrt((lam1[q] + 2.0f * mu1[q]) / d1[q]);
+  for (int k = 1; k <= nzt; ++k) {
+    const size_t q0 = (size_t)(k - 1) * plane;
+    const float l = lam1[q0], m = mu1[q0], d = d1[q0];
+    for (size_t q = q0; q < q0 + plane; ++q)
+      if (lam1[q] != l || mu1[q] != m || d1[q] != d)
+        return fail(h, FD3D_E_INVALID, "the medium must be constant over each z plane (depth-layered, as "
+                                        "fd3d_init builds it from crustal.dat); plane k = " + std::to_string(k) +
+                                        " varies");
+    if (!(d > 0.0f)) return fail(h, FD3D_E_INVALID, "density must be positive");
+    const float vp = std::sqrt((l + 2.0f * m) / d);
     if (vp > vmax) vmax = vp;
+    L[k] = l; M[k] = m; D[k] = d;
   }
   const float cfl = vmax * h->cfg.dt / h->cfg.dh;
   if (cfl > 0.25f) return fail(h, FD3D_E_CFL, "numerically unstable: CFL = " + std::to_string(cfl));
-  int rc;
-  if ((rc = upload3(h, h->p.f[0].lam, lam1))) return rc;
-  if ((rc = upload3(h, h->p.f[0].mu, mu1))) return rc;
-  if ((rc = upload3(h, h->p.f[0].d, d1))) return rc;
-  std::vector<float>().swap(h->staging);
+  const Fields &f = h->p.f[0];
+  HIPCHK(h, hipMemcpy(f.lam, L, ((size_t)nzt + 2) * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(f.mu, M, ((size_t)nzt + 2) * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(h, hipMemcpy(f.d, D, ((size_t)nzt + 2) * sizeof(float), hipMemcpyHostToDevice));
   h->medium_ok = true;
   return 0;
 }

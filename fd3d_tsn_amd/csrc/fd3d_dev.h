@@ -1,14 +1,17 @@
 // Device-side data layout of one rupture problem and the launch wrappers
 // implemented in kernels.hip.
 //
-// HBM layout.  Every 3D field (u, v, w, xx, yy, zz, xy, yz, xz, lam1, mu1,
-// d1) is its own float array in the reference's column-major order with a
-// padded x pitch: element (i,j,k), 1-based, lives at
+// HBM layout.  Every 3D wavefield array (u, v, w, xx, yy, zz, xy, yz, xz,
+// two sets) is its own float array in the reference's column-major order
+// with a padded x pitch: element (i,j,k), 1-based, lives at
 //   off0 + (i-1) + px*((j-1) + nyt*(k-1)),
 // where off0 shifts each row so that the first interior point i = nabc+1
 // starts a 256-byte boundary and px is a multiple of 64 floats.  A
 // wavefront (64 lanes along x) then reads whole 128-byte lines.  The fault
-// plane arrays (nxt, nzt) are unpadded.  PML split fields live in one pool,
+// plane arrays (nxt, nzt) are unpadded.  The medium is depth-layered, as
+// the reference builds it (lam1, mu1, d1 constant over each z plane,
+// fd3d_init.f90:240-265): lam, mu and rho are tables of one value per
+// plane, index k.  PML split fields live in one pool,
 // region by region, split by split, each split a dense (ni, nj, nk) box.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -39,7 +42,8 @@ struct Coef {
 };
 
 struct Fields {
-  float *u, *v, *w, *xx, *yy, *zz, *xy, *yz, *xz, *lam, *mu, *d;
+  float *u, *v, *w, *xx, *yy, *zz, *xy, *yz, *xz;
+  float *lam, *mu, *d;   // per-plane medium tables, index k = 1..nzt (0 and nzt+1 hold 0)
 };
 
 struct Fault {

@@ -103,7 +103,7 @@ __device__ inline void vp_update(const VelPos &P, const Dims &g, const Coef &c, 
     un = fn.u[q]; vn = fn.v[q]; wn = fn.w[q];
     return;
   }
-  const float uo = fo.u[q], vo = fo.v[q], wo = fo.w[q], dq = fo.d[q];
+  const float uo = fo.u[q], vo = fo.v[q], wo = fo.w[q], dq = fo.d[P.k];
   const float c1 = c.c1, c2 = c.c2;
   const float *Lxx = S[0], *Lxy = S[1], *Lxz = S[2], *Lyz = S[3], *Lzz = S[4];
   const int a0 = sat(x, z);
@@ -204,8 +204,8 @@ __global__ void __launch_bounds__(64 * FZE) k_vs_fused(Dims g, Coef c, Fields f0
       const int a0 = vat(tx, xz_z);
       const int q = I3(g, P.i, s, P.k);
       const float oxx = fo.xx[q], oyy = fo.yy[q], ozz = fo.zz[q], oxy = fo.xy[q], oxz = fo.xz[q], oyz = fo.yz[q];
-      const float xl = fo.lam[q], xm = fo.mu[q], mp1 = fo.mu[q + sy], mz1 = fo.mu[q + sz],
-                  mdiag = fo.mu[q + 1 + sy + sz];
+      // depth-layered medium: mu(j+1) = mu, mu(k+1) = mu(i+1, j+1, k+1)
+      const float xl = fo.lam[P.k], xm = fo.mu[P.k], mp1 = xm, mz1 = fo.mu[P.k + 1], mdiag = mz1;
       // own column: v(s-2..s+1) = vwm2.. (rows r-4..r-1); u, w (s-1..s+2) = rows r-3..r
       const float u0 = Lu[a0], v0 = Lv[a0], w0 = Lw[a0];
       const float vm2 = vwm4, vm1 = vwm3, vp1 = vwm1;

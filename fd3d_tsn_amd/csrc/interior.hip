@@ -20,7 +20,7 @@ __global__ void __launch_bounds__(TX * TZ) k_vel_tiled(Dims g, Coef c, Fields f0
 template <int TX, int TZ>
 __global__ void __launch_bounds__(TX * TZ) k_str_tiled(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl, int gx,
                                                        int gz, int gy, int NY) {
-  __shared__ float lds[2 * 4 * Tile<TX, TZ>::N];
+  __shared__ float lds[2 * 3 * Tile<TX, TZ>::N];
   const int lid = xcd_remap(blockIdx.x, gx * gz * gy);
   const FieldPair FP = field_pair(f0, f1, ctl);
   str_tiled_body<TX, TZ>(g, c, FP.fo, FP.fn, lds, lid % gx, (lid / gx) % gz, lid / (gx * gz), threadIdx.x, threadIdx.y, NY);

@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_vel_zm(Dims g, Coef c, Fields
   float yzm1 = fo.yz[q - sz], yz0 = fo.yz[q];
   for (int k2 = k2b;; ++k2) {
     const float az = amz[k2], bz = apz[k2];
-    const float dtd = dt / fo.d[q];
+    const float dtd = dt / fo.d[R.kb + k2 - 1];
     float pt, s1, s2, s3;
     s1 = s[0]; s2 = s[nc]; s3 = s[2 * nc];
     pt = s1 * amx + (dtd * (fo.xx[q] - fo.xx[q - 1])) / dh; s1 = pt / apx;
@@ -114,13 +114,15 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields
   const long long nc = R.ncell, nxy = (long long)R.ni * R.nj;
   float *s = pm.splits + R.split_off + (i2 - 1) + (long long)R.ni * (j2 - 1) + nxy * (k2b - 1);
   const float dt = c.dt, dh = c.dh;
-  const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fo.mu;
+  const float *u = fn.u, *v = fn.v, *w = fn.w;
   int q = I3(g, i, j, R.kb + k2b - 1);
   float u0 = u[q], up1 = u[q + sz], v0 = v[q], vp1 = v[q + sz];
-  float wm1 = w[q - sz], w0 = w[q], m0 = mu[q], mp1 = mu[q + sz];
+  float wm1 = w[q - sz], w0 = w[q];
   for (int k2 = k2b;; ++k2) {
     const float az = amz[k2], bz = apz[k2];
-    const float xl = fo.lam[q], xm = m0;
+    // depth-layered medium: lam, mu of this plane and mu of the next
+    const int k = R.kb + k2 - 1;
+    const float xl = fo.lam[k], xm = fo.mu[k], mp1 = fo.mu[k + 1];
     const float a = xl + 2.0f * xm, b = xl;
     const float diff1 = u[q + 1] - u0, diff2 = v0 - v[q - sy], diff3 = w0 - wm1;
     float pt, s1, s2, s3;
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields
     fn.zz[q] = (s1 + s2) + s3;
     const bool zero_top = R.ztop && k2 == R.nk;
     if (has_v) {
-      const float xmu = (xm + mu[q + sy]) / 2.0f;
+      const float xmu = (xm + xm) / 2.0f;
       s1 = s[18 * nc]; s2 = s[19 * nc];
       pt = s1 * amyS + ((dt * xmu) * (u[q + sy] - u0)) / dh; s1 = pt / apyS;
       pt = s2 * amx + ((dt * xmu) * (v0 - v[q - 1])) / dh; s2 = pt / apx;
@@ -161,7 +163,7 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields
       s[20 * nc] = s1; s[21 * nc] = s2;
     }
     if (has_v) {
-      const float xmu = (xm + mu[q + 1 + sy + sz]) / 2.0f;
+      const float xmu = (xm + mp1) / 2.0f;
       s1 = s[22 * nc]; s2 = s[23 * nc];
       pt = s1 * amzS[k2] + ((dt * xmu) * (vp1 - v0)) / dh; s1 = pt / apzS[k2];
       pt = s2 * amyS + ((dt * xmu) * (w[q + sy] - w0)) / dh; s2 = pt / apyS;
@@ -175,7 +177,6 @@ __global__ void __launch_bounds__(kPmlBlock) k_pml_str_zm(Dims g, Coef c, Fields
     u0 = up1; up1 = u[q + sz];
     v0 = vp1; vp1 = v[q + sz];
     wm1 = w0; w0 = w[q];
-    m0 = mp1; mp1 = mu[q + sz];
   }
 }
 

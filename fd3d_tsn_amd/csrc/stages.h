@@ -46,7 +46,7 @@ __device__ inline void vel_interior_body(const Dims &g, const Coef &c, const Fie
   if (i > g.nxt - g.nabc || j > g.nyt - 2) return;
   const int q = I3(g, i, j, k), sy = g.sy, sz = g.sz;
   const float c1 = c.c1, c2 = c.c2;
-  const float b = c.dth / fo.d[q];
+  const float b = c.dth / fo.d[k];
   const float *xx = fo.xx, *yy = fo.yy, *zz = fo.zz, *xy = fo.xy, *yz = fo.yz, *xz = fo.xz;
   fn.u[q] = fo.u[q] + b * (c1 * (xx[q] - xx[q - 1]) + c2 * (xx[q + 1] - xx[q - 2]) +
                          c1 * (xy[q] - xy[q - sy]) + c2 * (xy[q + sy] - xy[q - 2 * sy]) +
@@ -107,7 +107,7 @@ __device__ inline void vel_fault_body(const Dims &g, const Coef &c, const Fields
   {
     const int q = qm;
     const float *xx = fo.xx, *yy = fo.yy, *zz = fo.zz, *xy = fo.xy, *yz = fo.yz, *xz = fo.xz;
-    const float b = c.dth / fo.d[q];
+    const float b = c.dth / fo.d[k];
     float pdx = ((xx[q] - xx[q - 1]) + (xy[q] - xy[q - sy])) + (xz[q] - xz[q - sz]);
     float a = pdx - fl.au1[p];
     fn.u[q] = fo.u[q] + b * (pdx + c.damp_s * a);
@@ -126,13 +126,13 @@ __device__ inline void vel_fault_body(const Dims &g, const Coef &c, const Fields
   // tasu1 / tasw1 at j = nyt
   const int q = qm + sy;
   if (!FVW || !fv.fvw) {
-    const float b = c.dth / fo.d[q];
+    const float b = c.dth / fo.d[k];
     fn.u[q] = fo.u[q] + (b * 2.0f) * (fl.tx[p] + fl.RFx[p]);
     fn.w[q] = fo.w[q] + b * (2.0f * (fl.tz[p] + fl.RFz[p]));
     return;
   }
   const double dthF = (double)c.dt / (double)c.dh;
-  const double dd = (double)fo.d[q];
+  const double dd = (double)fo.d[k];
   const double two_dth_d = 2.0 * (dthF / dd);
   // tasu1: X-node (fd3d_deriv.f90:1141-1162)
   {
@@ -209,7 +209,7 @@ __device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &
   const bool hasv = j <= R.je_v;
   // every load before the first store (the stores could alias the inputs
   // as far as the compiler knows, which would serialise the loads)
-  const float d0 = fo.d[q];
+  const float d0 = fo.d[k];
   const float su1 = s[0 * nc], su2 = s[1 * nc], su3 = s[2 * nc];
   const float sw1 = s[6 * nc], sw2 = s[7 * nc], sw3 = s[8 * nc];
   const float xx0 = fo.xx[q], xxm = fo.xx[q - 1], xy0 = fo.xy[q], xym = fo.xy[q - sy];
@@ -276,7 +276,7 @@ __device__ inline void fuvw_w_body(const Dims &g, const Fields &fo, const Fields
   if (i < 2 || i > g.nxt - 2 || j < 2 || j > g.nyt) return;
   const int ks = g.nzt - g.nfs;
   const int q = I3(g, i, j, ks), sz = g.sz, sy = g.sy;
-  const float xl = fn.lam[q + sz], xm = fn.mu[q + sz];
+  const float xl = fn.lam[ks + 1], xm = fn.mu[ks + 1];
   const float a = 2.0f * xl, b = xl + 2.0f * xm;
   const float *u = fn.u, *v = fn.v;
   const float s = ((((((u[q + 1] - u[q]) + u[q + 1 + sz]) - u[q + sz]) + v[q]) - v[q - sy]) +
@@ -297,7 +297,7 @@ __device__ inline void str_interior_body(const Dims &g, const Coef &c, const Fie
   const int q = I3(g, i, j, k), sy = g.sy, sz = g.sz;
   const float c1 = c.c1, c2 = c.c2, dth = c.dth;
   const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fo.mu;
-  const float xl = fo.lam[q], xm = mu[q];
+  const float xl = fo.lam[k], xm = mu[k];
   const float a = xl + 2.0f * xm, b = xl;
   const float dux1 = c1 * (u[q + 1] - u[q]), dux2 = c2 * (u[q + 2] - u[q - 1]);
   const float dvy1 = c1 * (v[q] - v[q - sy]), dvy2 = c2 * (v[q + sy] - v[q - 2 * sy]);
@@ -305,13 +305,13 @@ __device__ inline void str_interior_body(const Dims &g, const Coef &c, const Fie
   fn.xx[q] = (fo.xx[q] + (dth * a) * (dux1 + dux2)) + (dth * b) * (((dvy1 + dvy2) + dwz1) + dwz2);
   fn.yy[q] = (fo.yy[q] + (dth * a) * (dvy1 + dvy2)) + (dth * b) * (((dux1 + dux2) + dwz1) + dwz2);
   fn.zz[q] = (fo.zz[q] + (dth * a) * (dwz1 + dwz2)) + (dth * b) * (((dux1 + dux2) + dvy1) + dvy2);
-  float xmu = 0.5f * (xm + mu[q + sy]);
+  float xmu = 0.5f * (xm + mu[k]);
   fn.xy[q] = fo.xy[q] + (dth * xmu) * (((c1 * (u[q + sy] - u[q]) + c2 * (u[q + 2 * sy] - u[q - sy])) +
                                       c1 * (v[q] - v[q - 1])) + c2 * (v[q + 1] - v[q - 2]));
-  xmu = 0.5f * (xm + mu[q + sz]);
+  xmu = 0.5f * (xm + mu[k + 1]);
   fn.xz[q] = fo.xz[q] + (dth * xmu) * (((c1 * (u[q + sz] - u[q]) + c2 * (u[q + 2 * sz] - u[q - sz])) +
                                       c1 * (w[q] - w[q - 1])) + c2 * (w[q + 1] - w[q - 2]));
-  xmu = 0.5f * (xm + mu[q + 1 + sy + sz]);
+  xmu = 0.5f * (xm + mu[k + 1]);
   fn.yz[q] = fo.yz[q] + (dth * xmu) * (((c1 * (v[q + sz] - v[q]) + c2 * (v[q + 2 * sz] - v[q - sz])) +
                                       c1 * (w[q + sy] - w[q])) + c2 * (w[q + 2 * sy] - w[q - sy]));
 }
@@ -339,18 +339,18 @@ __device__ inline void str_fault_body(const Dims &g, const Coef &c, const Fields
   const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fo.mu;
   {
     const float dth = c.dth;
-    const float xl = fo.lam[q], xm = mu[q];
+    const float xl = fo.lam[k], xm = mu[k];
     const float a = xl + 2.0f * xm, b = xl;
     fn.xx[q] = (fo.xx[q] + (dth * a) * (u[q + 1] - u[q])) + (dth * b) * (((v[q] - v[q - sy]) + w[q]) - w[q - sz]);
     fn.yy[q] = (fo.yy[q] + (dth * a) * (v[q] - v[q - sy])) + (dth * b) * (((u[q + 1] - u[q]) + w[q]) - w[q - sz]);
     fn.zz[q] = (fo.zz[q] + (dth * a) * (w[q] - w[q - sz])) + (dth * b) * (((u[q + 1] - u[q]) + v[q]) - v[q - sy]);
     const float dthh = c.dthh;
-    float xmu = xm + mu[q + sy];
+    float xmu = xm + mu[k];
     const float xyn = fo.xy[q] + (dthh * xmu) * (((u[q + sy] - u[q]) + v[q]) - v[q - 1]);
     fn.xy[q] = xyn;
-    xmu = xm + mu[q + sz];
+    xmu = xm + mu[k + 1];
     fn.xz[q] = fo.xz[q] + (dthh * xmu) * (((u[q + sz] - u[q]) + w[q]) - w[q - 1]);
-    xmu = xm + mu[q + 1 + sy + sz];
+    xmu = xm + mu[k + 1];
     const float yzn = fo.yz[q] + (dthh * xmu) * (((v[q + sz] - v[q]) + w[q + sy]) - w[q]);
     fn.yz[q] = yzn;
     fn.xy[q + sy] = xyn;
@@ -359,12 +359,12 @@ __device__ inline void str_fault_body(const Dims &g, const Coef &c, const Fields
   {
     const int qf = q + sy;
     const float c1 = c.c1, c2 = c.c2;
-    float xmu = mu[qf] + mu[qf + sz];
+    float xmu = mu[k] + mu[k + 1];
     fn.xz[qf] = fo.xz[qf] + (c.dthh * xmu) *
                               (((c1 * (u[qf + sz] - u[qf]) + c2 * (u[qf + 2 * sz] - u[qf - sz])) +
                                 c1 * (w[qf] - w[qf - 1])) + c2 * (w[qf + 1] - w[qf - 2]));
     const float dth = c.dth;
-    const float xl = fo.lam[qf], xm = mu[qf];
+    const float xl = fo.lam[k], xm = mu[k];
     const float a = xl + 2.0f * xm, b = xl;
     const float diff1 = c1 * (u[qf + 1] - u[qf]) + c2 * (u[qf + 2] - u[qf - 1]);
     const float diff3 = c1 * (w[qf] - w[qf - sz]) + c2 * (w[qf + sz] - w[qf - 2 * sz]);
@@ -406,7 +406,7 @@ __device__ inline void pml_str_body(const Dims &g, const Coef &c, const Fields &
   const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fn.mu;
   const bool hxy = j <= R.je_v, hxz = j <= R.je_xz;
   // every load before the first store (see pml_vel_body)
-  const float xl = fn.lam[q], xm = mu[q];
+  const float xl = fn.lam[k], xm = mu[k];
   const float u0 = u[q], up = u[q + 1], v0 = v[q], vmy = v[q - sy], w0 = w[q], wmz = w[q - sz];
   const float ax = amxS[i2], bx = apxS[i2], ay = amy[j2], by = apy[j2], az = amz[k2], bz = apz[k2];
   const float sxx1 = s[9 * nc], sxx2 = s[10 * nc], sxx3 = s[11 * nc];
@@ -417,11 +417,11 @@ __device__ inline void pml_str_body(const Dims &g, const Coef &c, const Fields &
   float myz = 0, vz = 0, wy = 0, syz1 = 0, syz2 = 0;
   if (hxy || hxz) { pamx = amx[i2]; papx = apx[i2]; }
   if (hxy) {
-    mxy = mu[q + sy]; uy = u[q + sy]; vmx = v[q - 1]; sxy1 = s[18 * nc]; sxy2 = s[19 * nc];
+    mxy = mu[k]; uy = u[q + sy]; vmx = v[q - 1]; sxy1 = s[18 * nc]; sxy2 = s[19 * nc];
     pamyS = amyS[j2]; papyS = apyS[j2];
-    myz = mu[q + 1 + sy + sz]; vz = v[q + sz]; wy = w[q + sy]; syz1 = s[22 * nc]; syz2 = s[23 * nc];
+    myz = mu[k + 1]; vz = v[q + sz]; wy = w[q + sy]; syz1 = s[22 * nc]; syz2 = s[23 * nc];
   }
-  if (hxz) { mxz = mu[q + sz]; uz = u[q + sz]; wmx = w[q - 1]; sxz1 = s[20 * nc]; sxz2 = s[21 * nc]; }
+  if (hxz) { mxz = mu[k + 1]; uz = u[q + sz]; wmx = w[q - 1]; sxz1 = s[20 * nc]; sxz2 = s[21 * nc]; }
   if (hxy || hxz) { pamzS = amzS[k2]; papzS = apzS[k2]; }
   const float a = xl + 2.0f * xm, b = xl;
   const float diff1 = up - u0, diff2 = v0 - vmy, diff3 = w0 - wmz;
@@ -507,7 +507,7 @@ __device__ inline void traction_body(const Dims &g, const Coef &c, const Fields 
   float av = c.damp_s * (pdz - fl.avdz[p]);
   float rf = pdz + av;
   fl.RFz[p] = rf;
-  const float dd = fn.d[q];
+  const float dd = fn.d[k];
   if (!top_free)
     fl.tz[p] = -rf - ((0.5f * dd) * dht) * fn.w[q];
   else
@@ -725,7 +725,7 @@ __device__ inline void rates_body(const Dims &g, const Coef &c, const Fields &fn
       for (long long n = t; n < slab; n += 256) {
         const int i = g.nabc + 1 + (int)(n % g.nxT), k = g.nabc + 1 + (int)(n / g.nxT);
         const float sx = srx[n], sz = srz[n];
-        const float m = fn.mu[I3(g, i, g.nyt, k)];
+        const float m = fn.mu[k];
         acc += (double)((((sqrtf(sx * sx + sz * sz) * m) * c.dh) * c.dh) / per);
       }
       red[t] = acc;

@@ -54,6 +54,8 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
   const float *xx = fo.xx, *yy = fo.yy, *zz = fo.zz, *xy = fo.xy, *yz = fo.yz, *xz = fo.xz;
   float xym2 = 0, xym1 = 0, xy0 = 0, xyp1 = 0, yym1 = 0, yy0 = 0, yyp1 = 0, yyp2 = 0;
   float yzm2 = 0, yzm1 = 0, yz0 = 0, yzp1 = 0;
+  // the medium is depth-layered: b = dt/(dh*rho) is one value per z row
+  const float b = c.dth / fo.d[active ? k : 0];
   if (inside) {
     const int q = I3(g, i, j0, k);
     xym2 = xy[q - 2 * sy]; xym1 = xy[q - sy]; xy0 = xy[q]; xyp1 = xy[q + sy];
@@ -75,8 +77,8 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
     const bool adv = j < je && inside;
     float nxy = 0, nyy = 0, nyz = 0;
     if (adv) { nxy = xy[q + 2 * sy]; nyy = yy[q + 3 * sy]; nyz = yz[q + 2 * sy]; }
-    float uo = 0, vo = 0, wo = 0, dq = 1;
-    if (active) { uo = ld_nt(&fo.u[q]); vo = ld_nt(&fo.v[q]); wo = ld_nt(&fo.w[q]); dq = ld_nt(&fo.d[q]); }
+    float uo = 0, vo = 0, wo = 0;
+    if (active) { uo = ld_nt(&fo.u[q]); vo = ld_nt(&fo.v[q]); wo = ld_nt(&fo.w[q]); }
     L[0][a0] = xx0; L[1][a0] = xy0; L[2][a0] = xz0; L[3][a0] = yz0; L[4][a0] = zz0;
     for (int h = t; h < NHTOT; h += TX * TZ) {
       int fld, hx, hz;
@@ -103,7 +105,6 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
     if (active) {
       const float *Lxx = L[0], *Lxy = L[1], *Lxz = L[2], *Lyz = L[3], *Lzz = L[4];
       const int W = T::W;
-      const float b = c.dth / dq;
       st_nt(&fn.u[q], uo + b * (c1 * (xx0 - Lxx[a0 - 1]) + c2 * (Lxx[a0 + 1] - Lxx[a0 - 2]) +
                              c1 * (xy0 - xym1) + c2 * (xyp1 - xym2) +
                              c1 * (xz0 - Lxz[a0 - W]) + c2 * (Lxz[a0 + W] - Lxz[a0 - 2 * W])));
@@ -125,7 +126,9 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
 
 // ---------------------------------------------------------------------------
 // Stress.  Register windows along y: v (j-2..j+1), u (j-1..j+2),
-// w (j-1..j+2), mu (j, j+1).  LDS x-z plane at row j: u, v, w, mu.
+// w (j-1..j+2).  LDS x-z plane at row j: u, v, w.  The medium is
+// depth-layered, so lam and mu of a lane's row k (and mu of row k+1, for
+// the xz and yz averages) are constants of the march.
 // ---------------------------------------------------------------------------
 template <int TX, int TZ>
 __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, float *lds,
@@ -141,35 +144,39 @@ __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields
   const bool inside = i <= g.nxt && k <= g.nzt;
   const int sy = g.sy, sz = g.sz;
   const float c1 = c.c1, c2 = c.c2, dth = c.dth;
-  const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fo.mu;
+  const float *u = fn.u, *v = fn.v, *w = fn.w;
+  // the reference's factors, formed once: (dth*(lam+2mu)), (dth*lam) and
+  // dth * 0.5*(mu + mu') for xy (mu' = mu(j+1) = mu), xz (mu(k+1)) and yz
+  // (mu(i+1,j+1,k+1) = mu(k+1))
+  const int km = active ? k : 0;
+  const float xl = fo.lam[km], xm = fo.mu[km], mz = fo.mu[km + 1];
+  const float da = dth * (xl + 2.0f * xm), db = dth * xl;
+  const float dxy = dth * (0.5f * (xm + xm)), dxz = dth * (0.5f * (xm + mz));
   float vm2 = 0, vm1 = 0, v0 = 0, vp1 = 0, um1 = 0, u0 = 0, up1 = 0, up2 = 0;
-  float wm1 = 0, w0 = 0, wp1 = 0, wp2 = 0, m0 = 0, mp1 = 0;
+  float wm1 = 0, w0 = 0, wp1 = 0, wp2 = 0;
   if (inside) {
     const int q = I3(g, i, j0, k);
     vm2 = v[q - 2 * sy]; vm1 = v[q - sy]; v0 = v[q]; vp1 = v[q + sy];
     um1 = u[q - sy]; u0 = u[q]; up1 = u[q + sy]; up2 = u[q + 2 * sy];
     wm1 = w[q - sy]; w0 = w[q]; wp1 = w[q + sy]; wp2 = w[q + 2 * sy];
-    m0 = mu[q]; mp1 = mu[q + sy];
   }
   // halo cells per field: x halos 3 columns x TZ rows, z halos 3 rows x TX
-  // (mu: one row)
-  constexpr int NXH = 3 * TZ, NZH = 3 * TX, NHTOT = 3 * NXH + 3 * NZH + TX;
+  constexpr int NXH = 3 * TZ, NZH = 3 * TX, NHTOT = 3 * NXH + 3 * NZH;
   int buf = 0;
   for (int j = j0; j <= je; ++j) {
     const int qj = I3(g, 0, j, 1);
-    float(*L)[T::N] = reinterpret_cast<float(*)[T::N]>(lds + buf * 4 * T::N);
+    float(*L)[T::N] = reinterpret_cast<float(*)[T::N]>(lds + buf * 3 * T::N);
     const int a0 = T::at(tx, tz);
     const int q = qj + i + (k - 1) * sz;
-    L[0][a0] = u0; L[1][a0] = v0; L[2][a0] = w0; L[3][a0] = m0;
+    L[0][a0] = u0; L[1][a0] = v0; L[2][a0] = w0;
     // every load of this row is issued before the stores below: the next
-    // rows of the y windows, the cell's old stresses, lam and the diagonal mu
+    // rows of the y windows and the cell's old stresses
     const bool adv = j < je && inside;
-    float nv = 0, nu = 0, nw = 0, nm = 0;
-    if (adv) { nv = v[q + 2 * sy]; nu = u[q + 3 * sy]; nw = w[q + 3 * sy]; nm = mu[q + 2 * sy]; }
-    float oxx = 0, oyy = 0, ozz = 0, oxy = 0, oxz = 0, oyz = 0, xl = 0, mdiag = 0;
+    float nv = 0, nu = 0, nw = 0;
+    if (adv) { nv = v[q + 2 * sy]; nu = u[q + 3 * sy]; nw = w[q + 3 * sy]; }
+    float oxx = 0, oyy = 0, ozz = 0, oxy = 0, oxz = 0, oyz = 0;
     if (active) {
       oxx = fo.xx[q]; oyy = fo.yy[q]; ozz = fo.zz[q]; oxy = fo.xy[q]; oxz = fo.xz[q]; oyz = fo.yz[q];
-      xl = fo.lam[q]; mdiag = mu[q + 1 + sy + sz];
     }
     for (int h = t; h < NHTOT; h += TX * TZ) {
       int fld, hx, hz;
@@ -179,51 +186,40 @@ __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields
         fld = f;
         hz = r;
         hx = f == 0 ? (ci == 0 ? -1 : TX + ci - 1) : (ci == 2 ? TX : ci - 2);
-      } else if (h < 3 * NXH + 3 * NZH) {
+      } else {
         // z halos of u, v (rows -1, TZ, TZ+1) and w (-2, -1, TZ)
         const int e0 = h - 3 * NXH, f = e0 / NZH, e = e0 - f * NZH, r = e / TX;
         fld = f;
         hx = e - r * TX;
         hz = f < 2 ? (r == 0 ? -1 : TZ + r - 1) : (r == 2 ? TZ : r - 2);
-      } else {
-        // mu, row TZ
-        fld = 3;
-        hx = h - 3 * NXH - 3 * NZH;
-        hz = TZ;
       }
       const int gi = i0 + hx, gk = kk0 + hz;
-      const float *s = fld == 0 ? u : fld == 1 ? v : fld == 2 ? w : mu;
+      const float *s = fld == 0 ? u : fld == 1 ? v : w;
       float val = 0.0f;
       if (gi >= 1 && gi <= g.nxt && gk >= 1 && gk <= g.nzt) val = s[qj + gi + (gk - 1) * sz];
       L[fld][T::at(hx, hz)] = val;
     }
     __syncthreads();
     if (active) {
-      const float *Lu = L[0], *Lv = L[1], *Lw = L[2], *Lm = L[3];
+      const float *Lu = L[0], *Lv = L[1], *Lw = L[2];
       const int W = T::W;
-      const float xm = m0;
-      const float a = xl + 2.0f * xm, b = xl;
       const float dux1 = c1 * (Lu[a0 + 1] - u0), dux2 = c2 * (Lu[a0 + 2] - Lu[a0 - 1]);
       const float dvy1 = c1 * (v0 - vm1), dvy2 = c2 * (vp1 - vm2);
       const float dwz1 = c1 * (w0 - Lw[a0 - W]), dwz2 = c2 * (Lw[a0 + W] - Lw[a0 - 2 * W]);
-      fn.xx[q] = (oxx + (dth * a) * (dux1 + dux2)) + (dth * b) * (((dvy1 + dvy2) + dwz1) + dwz2);
-      fn.yy[q] = (oyy + (dth * a) * (dvy1 + dvy2)) + (dth * b) * (((dux1 + dux2) + dwz1) + dwz2);
-      fn.zz[q] = (ozz + (dth * a) * (dwz1 + dwz2)) + (dth * b) * (((dux1 + dux2) + dvy1) + dvy2);
-      float xmu = 0.5f * (xm + mp1);
-      fn.xy[q] = oxy + (dth * xmu) * (((c1 * (up1 - u0) + c2 * (up2 - um1)) + c1 * (v0 - Lv[a0 - 1])) +
-                                         c2 * (Lv[a0 + 1] - Lv[a0 - 2]));
-      xmu = 0.5f * (xm + Lm[a0 + W]);
-      fn.xz[q] = oxz + (dth * xmu) * (((c1 * (Lu[a0 + W] - u0) + c2 * (Lu[a0 + 2 * W] - Lu[a0 - W])) +
-                                          c1 * (w0 - Lw[a0 - 1])) + c2 * (Lw[a0 + 1] - Lw[a0 - 2]));
-      xmu = 0.5f * (xm + mdiag);
-      fn.yz[q] = oyz + (dth * xmu) * (((c1 * (Lv[a0 + W] - v0) + c2 * (Lv[a0 + 2 * W] - Lv[a0 - W])) +
-                                          c1 * (wp1 - w0)) + c2 * (wp2 - wm1));
+      fn.xx[q] = ((oxx + da * (dux1 + dux2)) + db * (((dvy1 + dvy2) + dwz1) + dwz2));
+      fn.yy[q] = ((oyy + da * (dvy1 + dvy2)) + db * (((dux1 + dux2) + dwz1) + dwz2));
+      fn.zz[q] = ((ozz + da * (dwz1 + dwz2)) + db * (((dux1 + dux2) + dvy1) + dvy2));
+      fn.xy[q] = (oxy + dxy * (((c1 * (up1 - u0) + c2 * (up2 - um1)) + c1 * (v0 - Lv[a0 - 1])) +
+                                    c2 * (Lv[a0 + 1] - Lv[a0 - 2])));
+      fn.xz[q] = (oxz + dxz * (((c1 * (Lu[a0 + W] - u0) + c2 * (Lu[a0 + 2 * W] - Lu[a0 - W])) +
+                                     c1 * (w0 - Lw[a0 - 1])) + c2 * (Lw[a0 + 1] - Lw[a0 - 2])));
+      fn.yz[q] = (oyz + dxz * (((c1 * (Lv[a0 + W] - v0) + c2 * (Lv[a0 + 2 * W] - Lv[a0 - W])) +
+                                     c1 * (wp1 - w0)) + c2 * (wp2 - wm1)));
     }
     if (adv) {
       vm2 = vm1; vm1 = v0; v0 = vp1; vp1 = nv;
       um1 = u0; u0 = up1; up1 = up2; up2 = nu;
       wm1 = w0; w0 = wp1; wp1 = wp2; wp2 = nw;
-      m0 = mp1; mp1 = nm;
     }
     buf ^= 1;
   }

@@ -100,6 +100,9 @@ size_t fd3d_footprint(const fd3d_config *cfg);
 int fd3d_device_memory(size_t *free_bytes, size_t *total_bytes);
 
 /* medium_com arrays lam1, mu1, d1 (fd3d_init.f90:262-264), nxt*nyt*nzt each.
+ * The reference fills them plane by plane from the layered crustal model
+ * (fd3d_init.f90:240-265); the device keeps one value per z plane, so a
+ * medium that varies within a plane is refused with FD3D_E_INVALID.
  * Returns FD3D_E_CFL when max(Vp)*dt/dh > 0.25 (fd3d_init.f90:274-278). */
 int fd3d_upload_medium(fd3d_handle *h, const float *lam1, const float *mu1, const float *d1);
 

@@ -237,3 +237,50 @@ def test_fused_rejected_with_free_surface():
     with capi.Solver(case) as s:
         with pytest.raises(capi.FD3DError):
             s.set_tuning(5, 16, 8, 16, 8)
+
+
+def layered_medium(case, layers=((5200.0, 3000.0, 2500.0), (5600.0, 3230.0, 2600.0), (6000.0, 3464.0, 2670.0))):
+    """lam1, mu1, d1 of a depth-layered crust: the planes k are split into
+    len(layers) bands, each filled with one (Vp, Vs, rho) as
+    readcrustalmodel fills a layer (fd3d_init.f90:240-265)."""
+    f = np.float32
+    lam = np.empty((case.nxt, case.nyt, case.nzt), np.float32, order="F")
+    mu = np.empty_like(lam)
+    d = np.empty_like(lam)
+    nb = len(layers)
+    for k in range(case.nzt):
+        vp, vs, rho = (f(x) for x in layers[min(nb - 1, k * nb // case.nzt)])
+        lam[:, :, k] = rho * (vp * vp - f(2.0) * (vs * vs))
+        mu[:, :, k] = rho * (vs * vs)
+        d[:, :, k] = rho
+    return lam, mu, d
+
+
+@pytest.mark.parametrize("fspace", [True, False], ids=["fs", "free"])
+def test_layered_medium_bit_exact(fspace):
+    """Three layers of different Vp, Vs, rho: the per-plane medium tables
+    give the reference's lam(k), mu(k), mu(k+1) averages in every sweep."""
+    nsteps = 60
+    case = spread_stations(C.tiny(ntfd=nsteps, friction="sw", fspace=fspace, nabc=5, n=(40, 14, 30), seed=11))
+    med = layered_medium(case)
+    say(f"[parity] layered medium fspace={fspace} grid {case.nxt}x{case.nyt}x{case.nzt} steps {nsteps}")
+    orc = O.Oracle(case, O.F32, medium=med)
+    ro = orc.step(nsteps)
+    with capi.Solver(case, medium=med) as s:
+        rg = s.step(nsteps)
+        fields = {n: s.field(n) for n in capi.FIELDS}
+    for n in OUTS:
+        assert np.array_equal(rg[n].astype(np.float64), ro[n]), n
+    assert np.array_equal(rg["seis"].astype(np.float64), ro["seis"])
+    for n in capi.FIELDS:
+        assert np.array_equal(fields[n].astype(np.float64), orc.field(n)), n
+
+
+def test_medium_varying_within_a_plane_is_refused():
+    case = C.tiny(ntfd=4, friction="sw", fspace=True)
+    lam, mu, d = C.material(case, np.float32)
+    mu[3, 2, case.nzt // 2] *= np.float32(1.01)
+    say("[parity] non-layered medium refused")
+    with pytest.raises(capi.FD3DError) as e:
+        capi.Solver(case, medium=(lam, mu, d))
+    assert e.value.code == -1 and "plane" in str(e.value)
