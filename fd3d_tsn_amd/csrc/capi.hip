@@ -215,7 +215,7 @@ int validate(const fd3d_config *c) {
       return FD3D_E_INVALID;
   }
   Dims g = make_dims(*c);
-  if (g.n3 >= (1LL << 31)) return FD3D_E_INVALID;  // 32-bit cell indices
+  if (g.n3 >= (1LL << 30)) return FD3D_E_INVALID;  // 32-bit byte offsets into each 3D array
   return 0;
 }
 

@@ -32,6 +32,20 @@ __host__ __device__ inline int I3(const Dims &g, int i, int j, int k) {
 }
 __host__ __device__ inline int I2(const Dims &g, int i, int k) { return (i - 1) + g.nxt * (k - 1); }
 
+// Element access at a 32-bit element index from a wave-uniform base: the
+// memory unit adds the byte offset to the scalar base, so no 64-bit vector
+// address arithmetic is spent per access.  Every 3D array holds fewer than
+// 2^30 floats (checked at create), so byte offsets fit 32 bits.
+__device__ __forceinline__ float ld32(const float *base, int idx) {
+  return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + ((unsigned)idx << 2));
+}
+__device__ __forceinline__ void st32(float *base, int idx, float v) {
+  *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + ((unsigned)idx << 2)) = v;
+}
+__device__ __forceinline__ void st32_nt(float *base, int idx, float v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<float *>(reinterpret_cast<char *>(base) + ((unsigned)idx << 2)));
+}
+
 // Float constants formed on the host exactly as the reference forms them.
 struct Coef {
   float dh, dt;

@@ -2,240 +2,274 @@
 // and the stress update (dstres, :513-622) of the interior cells in one
 // pass over HBM.
 //
-// Separately, a step reads the six stresses twice (once for the
-// velocities, once to update them) and the velocities twice (update, then
-// stress stencils): 30 words per cell.  Fused, each of sigma_old, v_old,
-// rho, lam, mu is read once and v_new, sigma_new written once: 21 words.
-// The two wavefield sets make this race-free: every block reads the old
-// set and writes the new one.
+// Separately, the two sweeps move 27 words per interior cell (velocity: 6
+// old stresses and 3 old velocities read, 3 new velocities written;
+// stress: the 3 new velocities and 6 old stresses read, 6 written).  Fused,
+// the old stresses and velocities are read once and the new values written
+// once: 18 words.  The two wavefield sets make this race-free: every block
+// reads the old set and writes the new one.
 //
-// Work of a block: a 64 x FZC tile of the x-z plane (the core) marching
-// along y over the rows of its chunk.  Each row r:
-//   1. the old stresses of row r are staged in LDS over the core plus a
-//      4-cell ring (72 x FZC+8);
-//   2. new velocities of row r are computed over the core plus a 2-cell
-//      ring (68 x FZC+4), lanes on the ring recomputing what neighbouring
-//      tiles own; positions outside the interior (PML, near-fault rows,
-//      outer faces) take the values pml_vel / vel_fault already wrote;
-//      the plane goes into a 3-row LDS ring, core values also to HBM;
-//   3. new stresses of row r-2 are computed on the core from the LDS ring
-//      (x, z derivatives) and from per-lane register windows of the core
-//      column (y derivatives: v of rows r-4..r-1, u and w of r-3..r).
-// The march starts 2 rows before the chunk and ends 2 rows after it, so
-// each chunk recomputes 4 rows of velocities it does not own.
+// Work of a block (64 x 16 lanes).  Lane (tx, ty) owns position
+// x = tx - 2, z = ty - 2 of an x-z tile whose core, the cells that get new
+// stresses, is x in [0, 59], z in [0, 11]; the block marches along y over
+// the rows [j0, je] of its chunk, from j0 - 2 to je + 2.  Iteration r:
+//   1. the old-stress planes of row r (loaded into registers during
+//      iteration r - 1) go into one of two LDS buffers, and every lane puts
+//      its new velocity of row r - 2 (from its register window) into one of
+//      two velocity planes; one barrier;
+//   2. the loads of row r + 1 are issued; every lane forms the new velocity
+//      of its position at row r from the stress planes (x and z neighbours)
+//      and its register windows of xy, yy, yz (y neighbours); core lanes
+//      form the new stresses of row r - 2 from the velocity plane (x and z
+//      neighbours) and their velocity windows (y neighbours).
+// Lanes on the 2-cell ring around the core recompute velocities that the
+// neighbouring tiles own, so no block waits on another, and every lane
+// does the same work.  Positions outside the interior (PML cells, the
+// near-fault rows j >= nyt-1, the outer faces) take the values pml_vel and
+// vel_fault already wrote into the new set: the kernel runs after them and
+// before str_fault and pml_str, which read its new velocities.
 //
 // Per-point arithmetic is that of tiles.h (the same operands, the same
 // association), so the result is bit-identical to the separate sweeps.
 // Full-space problems only (with a free surface the ghost velocities of
 // fuvw must be formed between the two updates).
 #include "stages.h"
-#include "tiles.h"
 
 namespace fd3d {
 namespace {
 
-constexpr int FZE = 16;            // z rows of lanes (core + 2-cell ring)
-constexpr int FZC = FZE - 4;       // core z rows
-constexpr int SW = 64 + 8;         // stress plane: x in [-4, 67] relative to the tile
-constexpr int SH = FZC + 8;        //               z in [-4, FZC+3]
-constexpr int VW = 64 + 4;         // velocity plane: x in [-2, 65]
-constexpr int VH = FZE;            //                 z in [-2, FZC+1]
-constexpr int NSP = SW * SH;       // 1440
-constexpr int NVP = VW * VH;       // 1088
+// global-memory floats: keeps plane loads global_load when their field
+// pointer is chosen per wavefront
+typedef __attribute__((address_space(1))) const float gfloat;
 
-__device__ inline int sat(int x, int z) { return (z + 4) * SW + (x + 4); }
-__device__ inline int vat(int x, int z) { return (z + 2) * VW + (x + 2); }
+constexpr int LX = 64, LZ = 16;          // lanes along x and z
+constexpr int CX = LX - 4, CZ = LZ - 4;  // core: 60 x 12 cells
+constexpr int NT = LX * LZ;
 
-struct FusedLds {
-  float s[5][NSP];        // xx, xy, xz, yz, zz of row r
-  float v[3][3][NVP];     // ring of 3 rows x (u, v, w)
+// Old-stress planes of one row, relative to the core origin; every plane
+// spans what the velocity stencils of the lane positions x in [-2, 61],
+// z in [-2, 13] read.  The five planes lie back to back in one buffer,
+// each padded to whole wavefronts (64 cells) so that a wavefront's loads
+// of a plane row all come from one field.
+template <int X0, int Z0, int W_, int H_, int OFF>
+struct SP {
+  static constexpr int W = W_, N = W_ * H_, NW = (N + 63) / 64, off = OFF, end = OFF + 64 * NW;
+  static constexpr int x0 = X0, z0 = Z0;
+  __device__ static int at(int x, int z) { return OFF + (z - Z0) * W_ + (x - X0); }
+};
+using PXX = SP<-4, -2, 67, 16, 0>;          // xx: x in [-4, 62]
+using PXY = SP<-3, -2, 67, 16, PXX::end>;   // xy: x in [-3, 63]
+using PXZ = SP<-3, -4, 67, 19, PXY::end>;   // xz: x in [-3, 63], z in [-4, 14]
+using PYZ = SP<-2, -4, 64, 19, PXZ::end>;   // yz: z in [-4, 14]
+using PZZ = SP<-2, -3, 64, 19, PYZ::end>;   // zz: z in [-3, 15]
+constexpr int NSP = PZZ::end;               // 5888 floats
+constexpr int NWL = NSP / 64;               // wavefront loads per row (92)
+constexpr int NPF = (NWL + LZ - 1) / LZ;    // per wavefront (6)
+
+struct F2Lds {
+  float s[2][NSP];        // old stresses of row r (two buffers)
+  float v[2][3][NT];      // new u, v, w of row r - 2 at every lane position (two buffers)
 };
 
-// Velocity of one position (x, z relative to the tile) at row r: the
-// interior update where the position is an interior cell, else the value
-// already in the new set (0 outside the array).
-struct VelPos {
-  int i, k;                   // global indices
-  bool inside, interior_xz;   // in the array / interior in x and z
-  // y windows of the old stresses at this position
-  float xym2, xym1, xy0, xyp1, yym1, yy0, yyp1, yyp2, yzm2, yzm1, yz0, yzp1;
-};
-
-__device__ inline void vp_init(VelPos &P, const Dims &g, int i, int k) {
-  P.i = i; P.k = k;
-  P.inside = i >= 1 && i <= g.nxt && k >= 1 && k <= g.nzt;
-  P.interior_xz = i >= g.nabc + 1 && i <= g.nxt - g.nabc && k >= g.nabc + 1 && k <= g.nzt - g.nfs;
-  P.xym2 = P.xym1 = P.xy0 = P.xyp1 = 0.0f;
-  P.yym1 = P.yy0 = P.yyp1 = P.yyp2 = 0.0f;
-  P.yzm2 = P.yzm1 = P.yz0 = P.yzp1 = 0.0f;
+// Wavefront load m (wave-uniform) and lane -> field (0..4 = xx, xy, xz,
+// yz, zz), whether the cell is one of the plane's, and its position.
+template <class P>
+__device__ inline bool sp_pos(int e, int &x, int &z) {
+  z = e / P::W;
+  x = e - z * P::W + P::x0;
+  z += P::z0;
+  return e < P::N;
+}
+__device__ inline int sp_cell(int m, int lane, bool &ok, int &x, int &z) {
+  const int h = m * 64 + lane;
+  if (m < PXY::off / 64) { ok = sp_pos<PXX>(h - PXX::off, x, z); return 0; }
+  if (m < PXZ::off / 64) { ok = sp_pos<PXY>(h - PXY::off, x, z); return 1; }
+  if (m < PYZ::off / 64) { ok = sp_pos<PXZ>(h - PXZ::off, x, z); return 2; }
+  if (m < PZZ::off / 64) { ok = sp_pos<PYZ>(h - PYZ::off, x, z); return 3; }
+  ok = sp_pos<PZZ>(h - PZZ::off, x, z);
+  return 4;
 }
 
-// Old stress of field f at row j (0 outside the array).
-__device__ inline float ld_row(const float *f, const Dims &g, const VelPos &P, int j) {
-  return (P.inside && j >= 1 && j <= g.nyt) ? f[I3(g, P.i, j, P.k)] : 0.0f;
-}
-
-// Fill the y windows for row r: xy rows r-2..r+1, yy rows r-1..r+2, yz rows r-2..r+1.
-__device__ inline void vp_fill(VelPos &P, const Dims &g, const Fields &fo, int r) {
-  P.xym2 = ld_row(fo.xy, g, P, r - 2); P.xym1 = ld_row(fo.xy, g, P, r - 1);
-  P.xy0 = ld_row(fo.xy, g, P, r); P.xyp1 = ld_row(fo.xy, g, P, r + 1);
-  P.yym1 = ld_row(fo.yy, g, P, r - 1); P.yy0 = ld_row(fo.yy, g, P, r);
-  P.yyp1 = ld_row(fo.yy, g, P, r + 1); P.yyp2 = ld_row(fo.yy, g, P, r + 2);
-  P.yzm2 = ld_row(fo.yz, g, P, r - 2); P.yzm1 = ld_row(fo.yz, g, P, r - 1);
-  P.yz0 = ld_row(fo.yz, g, P, r); P.yzp1 = ld_row(fo.yz, g, P, r + 1);
-}
-
-// Advance the windows from row r to row r+1 (nxy, nyy, nyz: xy(r+2), yy(r+3), yz(r+2)).
-__device__ inline void vp_shift(VelPos &P, float nxy, float nyy, float nyz) {
-  P.xym2 = P.xym1; P.xym1 = P.xy0; P.xy0 = P.xyp1; P.xyp1 = nxy;
-  P.yym1 = P.yy0; P.yy0 = P.yyp1; P.yyp1 = P.yyp2; P.yyp2 = nyy;
-  P.yzm2 = P.yzm1; P.yzm1 = P.yz0; P.yz0 = P.yzp1; P.yzp1 = nyz;
-}
-
-// New (u, v, w) at position (x, z) of the tile, row r.  S is the LDS stress
-// plane of row r.  Arithmetic as vel_tiled_body.
-__device__ inline void vp_update(const VelPos &P, const Dims &g, const Coef &c, const Fields &fo, const Fields &fn,
-                                 const float (*S)[NSP], int x, int z, int r, float &un, float &vn, float &wn) {
-  un = vn = wn = 0.0f;
-  if (!P.inside || r < 1 || r > g.nyt) return;
-  const int q = I3(g, P.i, r, P.k);
-  if (!(P.interior_xz && r >= g.nabc + 1 && r <= g.nyt - 2)) {
-    un = fn.u[q]; vn = fn.v[q]; wn = fn.w[q];
-    return;
-  }
-  const float uo = fo.u[q], vo = fo.v[q], wo = fo.w[q], dq = fo.d[P.k];
-  const float c1 = c.c1, c2 = c.c2;
-  const float *Lxx = S[0], *Lxy = S[1], *Lxz = S[2], *Lyz = S[3], *Lzz = S[4];
-  const int a0 = sat(x, z);
-  const float xx0 = Lxx[a0], xz0 = Lxz[a0], zz0 = Lzz[a0];
-  const float xy0 = P.xy0, yz0 = P.yz0;
-  const float b = c.dth / dq;
-  un = uo + b * (c1 * (xx0 - Lxx[a0 - 1]) + c2 * (Lxx[a0 + 1] - Lxx[a0 - 2]) +
-                 c1 * (xy0 - P.xym1) + c2 * (P.xyp1 - P.xym2) +
-                 c1 * (xz0 - Lxz[a0 - SW]) + c2 * (Lxz[a0 + SW] - Lxz[a0 - 2 * SW]));
-  vn = vo + b * (c1 * (Lxy[a0 + 1] - xy0) + c2 * (Lxy[a0 + 2] - Lxy[a0 - 1]) +
-                 c1 * (P.yyp1 - P.yy0) + c2 * (P.yyp2 - P.yym1) +
-                 c1 * (yz0 - Lyz[a0 - SW]) + c2 * (Lyz[a0 + SW] - Lyz[a0 - 2 * SW]));
-  wn = wo + b * (c1 * (Lxz[a0 + 1] - xz0) + c2 * (Lxz[a0 + 2] - Lxz[a0 - 1]) +
-                 c1 * (yz0 - P.yzm1) + c2 * (P.yzp1 - P.yzm2) +
-                 c1 * (Lzz[a0 + SW] - zz0) + c2 * (Lzz[a0 + 2 * SW] - Lzz[a0 - SW]));
-}
-
-__global__ void __launch_bounds__(64 * FZE) k_vs_fused(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl,
-                                                       int gx, int gz, int gy, int NY) {
-  __shared__ FusedLds L;
-  const FieldPair FP = field_pair(f0, f1, ctl);
-  const Fields &fo = FP.fo, &fn = FP.fn;
+__global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl, int gx,
+                                                       int gz, int gy, int NY) {
+  __shared__ F2Lds L;
+  // the old set (fo) is set it & 1, the new one the other (see field_pair);
+  // only the pointers this kernel uses are selected
+  const bool odd = ctl->it & 1;
   const int lid = xcd_remap(blockIdx.x, gx * gz * gy);
   const int bx = lid % gx, bz = (lid / gx) % gz, by = lid / (gx * gz);
-  const int tx = threadIdx.x, ty = threadIdx.y, t = ty * 64 + tx;
-  const int i0 = g.nabc + 1 + bx * 64, k0 = g.nabc + 1 + bz * FZC;
-  const int j0 = g.nabc + 1 + by * NY;
-  const int je = min(j0 + NY - 1, g.nyt - 2);
+  const int tx = threadIdx.x, ty = threadIdx.y, t = ty * LX + tx;
+  const int wv = __builtin_amdgcn_readfirstlane(ty);   // the wavefront (one lane row)
+  const int i0 = g.nabc + 1 + bx * CX, k0 = g.nabc + 1 + bz * CZ;   // core origin
+  const int j0 = g.nabc + 1 + by * NY, je = min(j0 + NY - 1, g.nyt - 2);
+  const int x = tx - 2, z = ty - 2, i = i0 + x, k = k0 + z;
+  const bool in_xz = i >= 1 && i <= g.nxt && k >= 1 && k <= g.nzt;
+  const bool int_xz = i >= g.nabc + 1 && i <= g.nxt - g.nabc && k >= g.nabc + 1 && k <= g.nzt - g.nfs;
+  const bool core = int_xz && x >= 0 && x < CX && z >= 0 && z < CZ;
+  const int sy = g.sy;
   const float c1 = c.c1, c2 = c.c2, dth = c.dth;
-  const int sy = g.sy, sz = g.sz;
-
-  // lane position in the velocity plane: x = tx, z = ty - 2
-  const int xz_z = ty - 2;
-  VelPos P;
-  vp_init(P, g, i0 + tx, k0 + xz_z);
-  const bool core = xz_z >= 0 && xz_z < FZC;
-  const bool own = core && P.interior_xz;            // this lane's cell gets new stresses
-  // wave 0 also covers the x ring: columns -2, -1, 64, 65, all 16 z rows
-  const bool ring = ty == 0;
-  const int hx = tx < 16 ? -2 : tx < 32 ? -1 : tx < 48 ? 64 : 65;
-  const int hz = (tx & 15) - 2;
-  VelPos H;
-  vp_init(H, g, i0 + hx, k0 + hz);
-
+  // depth-layered medium: the factors of the lane's plane k (interior lanes)
+  const int km = int_xz ? k : g.nabc + 1;
+  const float b = dth / f0.d[km];
+  const float xl = f0.lam[km], xm = f0.mu[km], mz = f0.mu[km + 1];
+  const float da = dth * (xl + 2.0f * xm), db = dth * xl;
+  const float dxy = dth * (0.5f * (xm + xm)), dxz = dth * (0.5f * (xm + mz));
+  const int qb = in_xz ? I3(g, i, 1, k) : 0;   // (i, 1, k); row j adds (j-1)*sy
   const int r0 = j0 - 2, r1 = je + 2;
-  vp_fill(P, g, fo, r0);
-  if (ring) vp_fill(H, g, fo, r0);
-  // register windows of the new velocities of the lane's own column
-  float vwm4 = 0, vwm3 = 0, vwm2 = 0, vwm1 = 0;   // v rows r-4..r-1
-  float uwm3 = 0, uwm2 = 0, uwm1 = 0;             // u rows r-3..r-1 (u(r) is new this row)
-  float wwm3 = 0, wwm2 = 0, wwm1 = 0;             // w rows r-3..r-1
+  const int jlo = g.nabc + 1, jhi = g.nyt - 2;  // interior rows
 
-  for (int r = r0; r <= r1; ++r) {
-    // ---- 1. old stresses of row r over the 72 x (FZC+8) plane
-    __syncthreads();   // the previous row's readers of L.s are done
-    for (int h = t; h < 5 * NSP; h += 64 * FZE) {
-      const int fld = h / NSP, e = h - fld * NSP;
-      const int x = e % SW - 4, z = e / SW - 4;
-      const int gi = i0 + x, gk = k0 + z;
-      const float *src = fld == 0 ? fo.xx : fld == 1 ? fo.xy : fld == 2 ? fo.xz : fld == 3 ? fo.yz : fo.zz;
-      float val = 0.0f;
-      if (gi >= 1 && gi <= g.nxt && gk >= 1 && gk <= g.nzt && r >= 1 && r <= g.nyt)
-        val = src[I3(g, gi, r, gk)];
-      L.s[fld][e] = val;
+#define F2_SEL(a) const float *o##a = odd ? f1.a : f0.a; float *n##a = odd ? f0.a : f1.a
+  F2_SEL(xx); F2_SEL(yy); F2_SEL(zz); F2_SEL(xy); F2_SEL(yz); F2_SEL(xz); F2_SEL(u); F2_SEL(v); F2_SEL(w);
+#undef F2_SEL
+  float *nu_ = nu, *nv_ = nv, *nw_ = nw;
+#define F2_COL(f, j) ((in_xz && (j) >= 1 && (j) <= g.nyt) ? ld32(f, qb + ((j) - 1) * sy) : 0.0f)
+// velocity source of row j: the old set where this kernel updates the
+// position, the new set where another kernel already did, 0 outside
+#define F2_VSRC(j, a, bb, cc)                                                      \
+  do {                                                                              \
+    a = bb = cc = 0.0f;                                                             \
+    if (in_xz && (j) >= 1 && (j) <= g.nyt) {                                        \
+      const int q_ = qb + ((j) - 1) * sy;                                           \
+      if (int_xz && (j) >= jlo && (j) <= jhi) { a = ld32(ou, q_); bb = ld32(ov, q_); cc = ld32(ow, q_); } \
+      else { a = ld32(nu_, q_); bb = ld32(nv_, q_); cc = ld32(nw_, q_); }            \
+    }                                                                               \
+  } while (0)
+// plane cells of row j into registers: wavefront wv takes wavefront loads
+// m = wv, wv + LZ, ...; each is one field (a uniform base pointer).  The
+// row-independent part of each cell's offset is formed once (-1: no cell).
+  int poff[NPF];
+  const gfloat *psrc[NPF];   // wave-uniform field of each load
+#pragma unroll
+  for (int n = 0; n < NPF; ++n) {
+    const int m = wv + n * LZ;
+    poff[n] = -1;
+    psrc[n] = (const gfloat *)oxx;
+    if (m < NWL) {
+      int px, pz;
+      bool ok;
+      const int f = sp_cell(m, tx, ok, px, pz);
+      psrc[n] = (const gfloat *)(f == 0 ? oxx : f == 1 ? oxy : f == 2 ? oxz : f == 3 ? oyz : ozz);
+      const int gi = i0 + px, gk = k0 + pz;
+      if (ok && gi >= 1 && gi <= g.nxt && gk >= 1 && gk <= g.nzt) poff[n] = I3(g, gi, 1, gk);
     }
-    // next rows of the y windows, issued with the plane loads
-    const float pxy = ld_row(fo.xy, g, P, r + 2), pyy = ld_row(fo.yy, g, P, r + 3),
-                pyz = ld_row(fo.yz, g, P, r + 2);
-    float hxy = 0, hyy = 0, hyz = 0;
-    if (ring) { hxy = ld_row(fo.xy, g, H, r + 2); hyy = ld_row(fo.yy, g, H, r + 3); hyz = ld_row(fo.yz, g, H, r + 2); }
-    __syncthreads();
-
-    // ---- 2. new velocities of row r over the 68 x FZE plane
-    const int slot = (r - r0) % 3;
-    float un, vn, wn;
-    vp_update(P, g, c, fo, fn, L.s, tx, xz_z, r, un, vn, wn);
-    L.v[slot][0][vat(tx, xz_z)] = un;
-    L.v[slot][1][vat(tx, xz_z)] = vn;
-    L.v[slot][2][vat(tx, xz_z)] = wn;
-    if (own && r >= j0 && r <= je) {
-      const int q = I3(g, P.i, r, P.k);
-      fn.u[q] = un; fn.v[q] = vn; fn.w[q] = wn;
-    }
-    if (ring) {
-      float hu, hv, hw;
-      vp_update(H, g, c, fo, fn, L.s, hx, hz, r, hu, hv, hw);
-      L.v[slot][0][vat(hx, hz)] = hu;
-      L.v[slot][1][vat(hx, hz)] = hv;
-      L.v[slot][2][vat(hx, hz)] = hw;
-    }
-    __syncthreads();
-
-    // ---- 3. new stresses of row s = r - 2 on the core
-    const int s = r - 2;
-    if (own && s >= j0 && s <= je) {
-      const int ps = (s - r0) % 3;
-      const float *Lu = L.v[ps][0], *Lv = L.v[ps][1], *Lw = L.v[ps][2];
-      const int a0 = vat(tx, xz_z);
-      const int q = I3(g, P.i, s, P.k);
-      const float oxx = fo.xx[q], oyy = fo.yy[q], ozz = fo.zz[q], oxy = fo.xy[q], oxz = fo.xz[q], oyz = fo.yz[q];
-      // depth-layered medium: mu(j+1) = mu, mu(k+1) = mu(i+1, j+1, k+1)
-      const float xl = fo.lam[P.k], xm = fo.mu[P.k], mp1 = xm, mz1 = fo.mu[P.k + 1], mdiag = mz1;
-      // own column: v(s-2..s+1) = vwm2.. (rows r-4..r-1); u, w (s-1..s+2) = rows r-3..r
-      const float u0 = Lu[a0], v0 = Lv[a0], w0 = Lw[a0];
-      const float vm2 = vwm4, vm1 = vwm3, vp1 = vwm1;
-      const float um1 = uwm3, up1 = uwm1, up2 = un;
-      const float wm1 = wwm3, wp1 = wwm1, wp2 = wn;
-      const float a = xl + 2.0f * xm, b = xl;
-      const int W = VW;
-      const float dux1 = c1 * (Lu[a0 + 1] - u0), dux2 = c2 * (Lu[a0 + 2] - Lu[a0 - 1]);
-      const float dvy1 = c1 * (v0 - vm1), dvy2 = c2 * (vp1 - vm2);
-      const float dwz1 = c1 * (w0 - Lw[a0 - W]), dwz2 = c2 * (Lw[a0 + W] - Lw[a0 - 2 * W]);
-      fn.xx[q] = (oxx + (dth * a) * (dux1 + dux2)) + (dth * b) * (((dvy1 + dvy2) + dwz1) + dwz2);
-      fn.yy[q] = (oyy + (dth * a) * (dvy1 + dvy2)) + (dth * b) * (((dux1 + dux2) + dwz1) + dwz2);
-      fn.zz[q] = (ozz + (dth * a) * (dwz1 + dwz2)) + (dth * b) * (((dux1 + dux2) + dvy1) + dvy2);
-      float xmu = 0.5f * (xm + mp1);
-      fn.xy[q] = oxy + (dth * xmu) * (((c1 * (up1 - u0) + c2 * (up2 - um1)) + c1 * (v0 - Lv[a0 - 1])) +
-                                      c2 * (Lv[a0 + 1] - Lv[a0 - 2]));
-      xmu = 0.5f * (xm + mz1);
-      fn.xz[q] = oxz + (dth * xmu) * (((c1 * (Lu[a0 + W] - u0) + c2 * (Lu[a0 + 2 * W] - Lu[a0 - W])) +
-                                       c1 * (w0 - Lw[a0 - 1])) + c2 * (Lw[a0 + 1] - Lw[a0 - 2]));
-      xmu = 0.5f * (xm + mdiag);
-      fn.yz[q] = oyz + (dth * xmu) * (((c1 * (Lv[a0 + W] - v0) + c2 * (Lv[a0 + 2 * W] - Lv[a0 - W])) +
-                                       c1 * (wp1 - w0)) + c2 * (wp2 - wm1));
-    }
-    // ---- advance the windows
-    vwm4 = vwm3; vwm3 = vwm2; vwm2 = vwm1; vwm1 = vn;
-    uwm3 = uwm2; uwm2 = uwm1; uwm1 = un;
-    wwm3 = wwm2; wwm2 = wwm1; wwm1 = wn;
-    vp_shift(P, pxy, pyy, pyz);
-    if (ring) vp_shift(H, hxy, hyy, hyz);
   }
+#define F2_PLANES(pf, j)                                                            \
+  do {                                                                              \
+    const bool jok_ = (j) >= 1 && (j) <= g.nyt;                                     \
+    const int jo_ = ((j) - 1) * sy;                                                 \
+    _Pragma("unroll") for (int n = 0; n < NPF; ++n)                                 \
+      pf[n] = (jok_ && poff[n] >= 0) ? *(const gfloat *)((const __attribute__((address_space(1))) char *)psrc[n] + ((unsigned)(poff[n] + jo_) << 2)) : 0.0f; \
+  } while (0)
+
+  // y windows of the old stresses: xy rows r-2..r, yy rows r-2..r+1, yz
+  // rows r-2..r before iteration r (its prefetch set adds xy(r+1),
+  // yy(r+2), yz(r+1))
+  float xym2 = 0, xym1 = 0, xy0 = 0, yym2 = 0, yym1 = 0, yy0 = 0, yyp1 = 0, yzm2 = 0, yzm1 = 0, yz0 = 0;
+  if (int_xz) {
+    xym2 = F2_COL(oxy, r0 - 2); xym1 = F2_COL(oxy, r0 - 1); xy0 = F2_COL(oxy, r0);
+    yym2 = F2_COL(oyy, r0 - 2); yym1 = F2_COL(oyy, r0 - 1); yy0 = F2_COL(oyy, r0); yyp1 = F2_COL(oyy, r0 + 1);
+    yzm2 = F2_COL(oyz, r0 - 2); yzm1 = F2_COL(oyz, r0 - 1); yz0 = F2_COL(oyz, r0);
+  }
+  // new velocities of the lane's column: v rows r-4..r-1 (vA..vD), u and w
+  // rows r-3..r-1 (uA..uC, wA..wC)
+  float vA = 0, vB = 0, vC = 0, vD = 0, uA = 0, uB = 0, uC = 0, wA = 0, wB = 0, wC = 0;
+  // old xx, xz, zz of the lane's position at rows r-1 and r-2
+  float xx1 = 0, xx2 = 0, xz1 = 0, xz2 = 0, zz1 = 0, zz2 = 0;
+  // Two prefetch sets, loaded two rows ahead and used alternately: the set
+  // of row R holds its plane cells, xy(R+1), yy(R+2), yz(R+1) of the lane's
+  // column and the velocity source of row R.
+  float pfA[NPF], pfB[NPF];
+  float qxyA = 0, qyyA = 0, qyzA = 0, quA, qvA, qwA, qxyB = 0, qyyB = 0, qyzB = 0, quB, qvB, qwB;
+#define F2_FETCH(P, R)                                                              \
+  do {                                                                              \
+    F2_PLANES(pf##P, R);                                                            \
+    if (int_xz) { qxy##P = F2_COL(oxy, (R) + 1); qyy##P = F2_COL(oyy, (R) + 2); qyz##P = F2_COL(oyz, (R) + 1); } \
+    F2_VSRC(R, qu##P, qv##P, qw##P);                                                \
+  } while (0)
+// Iteration r with prefetch set P.
+#define F2_ITER(P)                                                                  \
+  do {                                                                              \
+    const int buf = r & 1;                                                          \
+    float *S = L.s[buf];                                                            \
+    float(*V)[NT] = L.v[buf];                                                       \
+    /* 1. planes of row r into LDS; new velocity of row r-2 into the velocity plane */ \
+    _Pragma("unroll") for (int n = 0; n < NPF; ++n) {                               \
+      const int m = wv + n * LZ;                                                    \
+      if (m < NWL) S[m * 64 + tx] = pf##P[n];                                       \
+    }                                                                               \
+    V[0][t] = uB; V[1][t] = vC; V[2][t] = wB;                                       \
+    const float xyp1 = qxy##P, yyp2 = qyy##P, yzp1 = qyz##P;                        \
+    const float uo = qu##P, vo = qv##P, wo = qw##P;                                 \
+    __syncthreads();                                                                \
+    /* 2. loads of row r+2 into this set */                                         \
+    if (r + 2 <= r1) F2_FETCH(P, r + 2);                                            \
+    F2_COMPUTE();                                                                   \
+  } while (0)
+// New velocity of the lane's position at row r, new stresses of row r-2,
+// then the windows advance to row r+1.
+#define F2_COMPUTE()                                                                \
+  do {                                                                              \
+    float un = uo, vn = vo, wn = wo;                                                \
+    const float xx0 = S[PXX::at(x, z)], xz0 = S[PXZ::at(x, z)], zz0 = S[PZZ::at(x, z)]; \
+    if (int_xz && r >= jlo && r <= jhi) {                                           \
+      const int ax = PXX::at(x, z), ay = PXY::at(x, z), az = PXZ::at(x, z);          \
+      const int ayz = PYZ::at(x, z), azz = PZZ::at(x, z);                           \
+      constexpr int WXZ = PXZ::W, WYZ = PYZ::W, WZZ = PZZ::W;                       \
+      un = uo + b * (c1 * (xx0 - S[ax - 1]) + c2 * (S[ax + 1] - S[ax - 2]) +         \
+                     c1 * (xy0 - xym1) + c2 * (xyp1 - xym2) +                       \
+                     c1 * (xz0 - S[az - WXZ]) + c2 * (S[az + WXZ] - S[az - 2 * WXZ])); \
+      vn = vo + b * (c1 * (S[ay + 1] - xy0) + c2 * (S[ay + 2] - S[ay - 1]) +         \
+                     c1 * (yyp1 - yy0) + c2 * (yyp2 - yym1) +                       \
+                     c1 * (yz0 - S[ayz - WYZ]) + c2 * (S[ayz + WYZ] - S[ayz - 2 * WYZ])); \
+      wn = wo + b * (c1 * (S[az + 1] - xz0) + c2 * (S[az + 2] - S[az - 1]) +         \
+                     c1 * (yz0 - yzm1) + c2 * (yzp1 - yzm2) +                       \
+                     c1 * (S[azz + WZZ] - zz0) + c2 * (S[azz + 2 * WZZ] - S[azz - WZZ])); \
+      if (core && r >= j0 && r <= je) {                                             \
+        const int q = qb + (r - 1) * sy;                                            \
+        st32_nt(nu, q, un); st32_nt(nv, q, vn); st32_nt(nw, q, wn);                 \
+      }                                                                             \
+    }                                                                               \
+    const int s = r - 2;                                                            \
+    if (core && s >= j0 && s <= je) {                                               \
+      const float *Lu = V[0], *Lv = V[1], *Lw = V[2];                               \
+      const float u0 = uB, v0 = vC, w0 = wB;                                        \
+      const float dux1 = c1 * (Lu[t + 1] - u0), dux2 = c2 * (Lu[t + 2] - Lu[t - 1]); \
+      const float dvy1 = c1 * (v0 - vB), dvy2 = c2 * (vD - vA);                      \
+      const float dwz1 = c1 * (w0 - Lw[t - LX]), dwz2 = c2 * (Lw[t + LX] - Lw[t - 2 * LX]); \
+      const int q = qb + (s - 1) * sy;                                              \
+      st32(nxx, q, (xx2 + da * (dux1 + dux2)) + db * (((dvy1 + dvy2) + dwz1) + dwz2)); \
+      st32(nyy, q, (yym2 + da * (dvy1 + dvy2)) + db * (((dux1 + dux2) + dwz1) + dwz2)); \
+      st32(nzz, q, (zz2 + da * (dwz1 + dwz2)) + db * (((dux1 + dux2) + dvy1) + dvy2)); \
+      st32(nxy, q, xym2 + dxy * (((c1 * (uC - u0) + c2 * (un - uA)) + c1 * (v0 - Lv[t - 1])) + \
+                                   c2 * (Lv[t + 1] - Lv[t - 2])));                  \
+      st32(nxz, q, xz2 + dxz * (((c1 * (Lu[t + LX] - u0) + c2 * (Lu[t + 2 * LX] - Lu[t - LX])) + \
+                                   c1 * (w0 - Lw[t - 1])) + c2 * (Lw[t + 1] - Lw[t - 2]))); \
+      st32(nyz, q, yzm2 + dxz * (((c1 * (Lv[t + LX] - v0) + c2 * (Lv[t + 2 * LX] - Lv[t - LX])) + \
+                                    c1 * (wC - w0)) + c2 * (wn - wA)));             \
+    }                                                                               \
+    xx2 = xx1; xx1 = xx0; xz2 = xz1; xz1 = xz0; zz2 = zz1; zz1 = zz0;               \
+    vA = vB; vB = vC; vC = vD; vD = vn;                                             \
+    uA = uB; uB = uC; uC = un;                                                      \
+    wA = wB; wB = wC; wC = wn;                                                      \
+    xym2 = xym1; xym1 = xy0; xy0 = xyp1;                                            \
+    yym2 = yym1; yym1 = yy0; yy0 = yyp1; yyp1 = yyp2;                               \
+    yzm2 = yzm1; yzm1 = yz0; yz0 = yzp1;                                            \
+  } while (0)
+
+  F2_FETCH(A, r0);
+  if (r0 + 1 <= r1) F2_FETCH(B, r0 + 1);
+  for (int r = r0;;) {
+    F2_ITER(A);
+    if (++r > r1) break;
+    F2_ITER(B);
+    if (++r > r1) break;
+  }
+#undef F2_FETCH
+#undef F2_ITER
+#undef F2_COMPUTE
+#undef F2_COL
+#undef F2_VSRC
+#undef F2_PLANES
 }
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -247,9 +281,9 @@ hipError_t launch_fused(const Problem &p, hipStream_t s) {
   if (g.nyt - 2 - g.nabc <= 0) return hipSuccess;
   const int ny = p.tune.ny_vel;
   if (ny < 1) return hipErrorInvalidValue;
-  const int gx = cdiv(g.nxt - 2 * g.nabc, 64), gz = cdiv(g.nzt - g.nfs - g.nabc, FZC);
+  const int gx = cdiv(g.nxt - 2 * g.nabc, CX), gz = cdiv(g.nzt - g.nfs - g.nabc, CZ);
   const int gy = cdiv(g.nyt - 2 - g.nabc, ny);
-  hipLaunchKernelGGL(k_vs_fused, dim3(gx * gz * gy), dim3(64, FZE), 0, s, g, p.c, p.f[0], p.f[1], p.ctl, gx, gz, gy,
+  hipLaunchKernelGGL(k_vs_fused, dim3(gx * gz * gy), dim3(LX, LZ), 0, s, g, p.c, p.f[0], p.f[1], p.ctl, gx, gz, gy,
                      ny);
   return hipGetLastError();
 }
