@@ -103,6 +103,9 @@ int build_regions(const fd3d_config &c, const Dims &g, Pml &pm, long long &nspli
     R.ni = s.ie - s.ib + 1; R.nj = s.je_uw - s.jb + 1; R.nk = s.ke - s.kb + 1;
     if (R.ni <= 0 || R.nj <= 0 || R.nk <= 0) return -1;
     R.ncell = R.ni * R.nj * R.nk;
+    if ((long long)kSplits * R.ncell >= (1LL << 30)) return -1;   // 32-bit byte offsets into the split block
+    udiv_magic(R.ni, R.mdi, R.sdi);
+    udiv_magic(R.nj, R.mdj, R.sdj);
     R.ztop = s.ztop; R.nzx = s.nzx; R.nzy = s.nzy;
     R.blk0 = blk;
     blk += (R.ncell + kPmlBlock - 1) / kPmlBlock;
@@ -386,6 +389,7 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
   for (auto pp : d2)
     if ((rc = dalloc(h, pp, n2))) return bail(rc);
   if ((rc = dalloc(h, &p.pml.splits, (size_t)nsplit))) return bail(rc);
+  for (int r = 0; r < p.pml.nreg; ++r) p.pml.r[r].sp = p.pml.splits + p.pml.r[r].split_off;
   if ((rc = dalloc(h, &p.pml.prof, (size_t)nprof))) return bail(rc);
   const size_t slab = (size_t)p.g.nxT * p.g.nzT;
   if ((rc = dalloc(h, &p.out.ring, (size_t)h->cfg.ring_steps * FD3D_NOUT * slab))) return bail(rc);

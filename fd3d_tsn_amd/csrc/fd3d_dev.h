@@ -32,12 +32,29 @@ __host__ __device__ inline int I3(const Dims &g, int i, int j, int k) {
 }
 __host__ __device__ inline int I2(const Dims &g, int i, int k) { return (i - 1) + g.nxt * (k - 1); }
 
+// Quotient x / d for 0 <= x < 2^31 from m = ceil(2^(31+l) / d) and
+// sh = 31 + l, l = ceil(log2 d): with x * d < 2^(31+l) the rounding of m
+// never reaches the next integer, so the result is exact.
+__host__ __device__ inline int udiv(int x, unsigned m, int sh) {
+  return (int)(((unsigned long long)(unsigned)x * m) >> sh);
+}
+__host__ inline void udiv_magic(int d, unsigned &m, int &sh) {
+  int l = 0;
+  while ((1LL << l) < d) ++l;
+  sh = 31 + l;
+  m = (unsigned)(((1ULL << sh) + (unsigned long long)d - 1) / (unsigned long long)d);
+}
+
 // Element access at a 32-bit element index from a wave-uniform base: the
 // memory unit adds the byte offset to the scalar base, so no 64-bit vector
 // address arithmetic is spent per access.  Every 3D array holds fewer than
 // 2^30 floats (checked at create), so byte offsets fit 32 bits.
 __device__ __forceinline__ float ld32(const float *base, int idx) {
   return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + ((unsigned)idx << 2));
+}
+__device__ __forceinline__ float ld32_nt(const float *base, int idx) {
+  return __builtin_nontemporal_load(reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) +
+                                                                     ((unsigned)idx << 2)));
 }
 __device__ __forceinline__ void st32(float *base, int idx, float v) {
   *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + ((unsigned)idx << 2)) = v;
@@ -79,6 +96,9 @@ struct FvwScalars {
 struct PmlRegion {
   int ib, ie, jb, je_uw, je_v, je_xz, kb, ke;
   int ni, nj, nk, ncell;
+  unsigned mdi, mdj;       // x / ni = udiv(x, mdi, sdi) and x / nj likewise, for 0 <= x < 2^31
+  int sdi, sdj;
+  float *sp;               // the region's 24 splits, split n at sp + n * ncell
   int ztop, nzx, nzy;      // zero xz/yz splits at k2 = nk for j2 <= nzx / nzy
   int blk0;                // first block of this region in the fused PML launch
   int blkz0, nxyb, nzc;    // z-march launch: first block, (i,j) blocks, z chunks

@@ -182,45 +182,66 @@ __device__ inline int pml_region(const Pml &pm, int blk) {
   return r;
 }
 
+// Cell of a fused PML launch: region, (i2, j2, k2) in the region's box and
+// the 3D index.  The divisions by the box extents use host-formed
+// multipliers (PmlRegion::mdi, mdj).
+struct PmlCell {
+  int r, local, i2, j2, k2, i, j, k, q;
+};
+__device__ inline bool pml_cell(const Dims &g, const Pml &pm, int gbx, int lx, PmlCell &c) {
+  c.r = pml_region(pm, gbx);
+  const PmlRegion &R = pm.r[c.r];
+  c.local = (gbx - R.blk0) * kPmlBlock + lx;
+  if (c.local >= R.ncell) return false;
+  const int t = udiv(c.local, R.mdi, R.sdi);
+  const int km = udiv(t, R.mdj, R.sdj);
+  c.i2 = c.local - t * R.ni + 1;
+  c.j2 = t - km * R.nj + 1;
+  c.k2 = km + 1;
+  c.i = R.ib + c.i2 - 1;
+  c.j = R.jb + c.j2 - 1;
+  c.k = R.kb + c.k2 - 1;
+  c.q = I3(g, c.i, c.j, c.k);
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // Split-field PML velocities, all regions in one launch (pml_uvw,
 // fd3d_deriv.f90:1927-2299): p <- [p(1-w) + ((dt/rho)*dsigma)/dh]/(1+w),
-// field = sum of splits.  u and w to j = je_uw, v to je_v.
+// field = sum of splits.  u and w to j = je_uw, v to je_v.  All accesses
+// are 32-bit offsets from scalar bases (the region's split block, the
+// wavefield arrays, the profile pool).
 // ---------------------------------------------------------------------------
 __device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, const Pml &pm, int gbx, int lx) {
-  const int r = pml_region(pm, gbx);
-  const PmlRegion &R = pm.r[r];
-  const int local = (gbx - R.blk0) * kPmlBlock + lx;
-  if (local >= R.ncell) return;
-  const int i2 = local % R.ni + 1;
-  const int t = local / R.ni;
-  const int j2 = t % R.nj + 1;
-  const int k2 = t / R.nj + 1;
-  const int i = R.ib + i2 - 1, j = R.jb + j2 - 1, k = R.kb + k2 - 1;
-  const int q = I3(g, i, j, k), sy = g.sy, sz = g.sz;
-  const float *pr = pm.prof + R.prof_off;
+  PmlCell C;
+  if (!pml_cell(g, pm, gbx, lx, C)) return;
+  const PmlRegion &R = pm.r[C.r];
+  const int q = C.q, sy = g.sy, sz = g.sz, i2 = C.i2, j2 = C.j2, k2 = C.k2;
+  const float *pr = pm.prof;
   const int nI = R.ni + 1, nJ = R.nj + 1, nK = R.nk + 1;
-  const float *amx = pr, *apx = pr + nI, *amxS = pr + 2 * nI, *apxS = pr + 3 * nI;
-  const float *amy = pr + 4 * nI, *apy = amy + nJ;
-  const float *amz = pr + 4 * nI + 4 * nJ, *apz = amz + nK;
-  float *s = pm.splits + R.split_off + local;
-  const long long nc = R.ncell;
+  const int amx = R.prof_off, apx = amx + nI, amxS = amx + 2 * nI, apxS = amx + 3 * nI;
+  const int amy = amx + 4 * nI, apy = amy + nJ;
+  const int amz = amx + 4 * nI + 4 * nJ, apz = amz + nK;
+  float *sp = R.sp;
+  const int nc = R.ncell, l = C.local;
   const float dt = c.dt, dh = c.dh;
-  const bool hasv = j <= R.je_v;
+  const bool hasv = C.j <= R.je_v;
   // every load before the first store (the stores could alias the inputs
   // as far as the compiler knows, which would serialise the loads)
-  const float d0 = fo.d[k];
-  const float su1 = s[0 * nc], su2 = s[1 * nc], su3 = s[2 * nc];
-  const float sw1 = s[6 * nc], sw2 = s[7 * nc], sw3 = s[8 * nc];
-  const float xx0 = fo.xx[q], xxm = fo.xx[q - 1], xy0 = fo.xy[q], xym = fo.xy[q - sy];
-  const float xz0 = fo.xz[q], xzmz = fo.xz[q - sz], xzp = fo.xz[q + 1];
-  const float yz0 = fo.yz[q], yzmy = fo.yz[q - sy], zz0 = fo.zz[q], zzp = fo.zz[q + sz];
-  const float pamx = amx[i2], papx = apx[i2], pamxS = amxS[i2], papxS = apxS[i2];
-  const float pamy = amy[j2], papy = apy[j2], pamz = amz[k2], papz = apz[k2];
+  const float d0 = fo.d[C.k];
+  const float su1 = ld32(sp, l), su2 = ld32(sp, l + nc), su3 = ld32(sp, l + 2 * nc);
+  const float sw1 = ld32(sp, l + 6 * nc), sw2 = ld32(sp, l + 7 * nc), sw3 = ld32(sp, l + 8 * nc);
+  const float xx0 = ld32(fo.xx, q), xxm = ld32(fo.xx, q - 1), xy0 = ld32(fo.xy, q), xym = ld32(fo.xy, q - sy);
+  const float xz0 = ld32(fo.xz, q), xzmz = ld32(fo.xz, q - sz), xzp = ld32(fo.xz, q + 1);
+  const float yz0 = ld32(fo.yz, q), yzmy = ld32(fo.yz, q - sy), zz0 = ld32(fo.zz, q), zzp = ld32(fo.zz, q + sz);
+  const float pamx = ld32(pr, amx + i2), papx = ld32(pr, apx + i2), pamxS = ld32(pr, amxS + i2),
+              papxS = ld32(pr, apxS + i2);
+  const float pamy = ld32(pr, amy + j2), papy = ld32(pr, apy + j2), pamz = ld32(pr, amz + k2),
+              papz = ld32(pr, apz + k2);
   float sv1 = 0, sv2 = 0, sv3 = 0, xyp = 0, yy0 = 0, yyp = 0, yzmz = 0;
   if (hasv) {
-    sv1 = s[3 * nc]; sv2 = s[4 * nc]; sv3 = s[5 * nc];
-    xyp = fo.xy[q + 1]; yy0 = fo.yy[q]; yyp = fo.yy[q + sy]; yzmz = fo.yz[q - sz];
+    sv1 = ld32(sp, l + 3 * nc); sv2 = ld32(sp, l + 4 * nc); sv3 = ld32(sp, l + 5 * nc);
+    xyp = ld32(fo.xy, q + 1); yy0 = ld32(fo.yy, q); yyp = ld32(fo.yy, q + sy); yzmz = ld32(fo.yz, q - sz);
   }
   const float dtd = dt / d0;
   float pt, s1, s2, s3;
@@ -231,8 +252,8 @@ __device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &
   s2 = pt / papy;
   pt = su3 * pamz + (dtd * (xz0 - xzmz)) / dh;
   s3 = pt / papz;
-  s[0 * nc] = s1; s[1 * nc] = s2; s[2 * nc] = s3;
-  st_nt(&fn.u[q], (s1 + s2) + s3);
+  st32(sp, l, s1); st32(sp, l + nc, s2); st32(sp, l + 2 * nc, s3);
+  st32_nt(fn.u, q, (s1 + s2) + s3);
   // w (xS, y, z profiles)
   pt = sw1 * pamxS + (dtd * (xzp - xz0)) / dh;
   s1 = pt / papxS;
@@ -240,8 +261,8 @@ __device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &
   s2 = pt / papy;
   pt = sw3 * pamz + (dtd * (zzp - zz0)) / dh;
   s3 = pt / papz;
-  s[6 * nc] = s1; s[7 * nc] = s2; s[8 * nc] = s3;
-  st_nt(&fn.w[q], (s1 + s2) + s3);
+  st32(sp, l + 6 * nc, s1); st32(sp, l + 7 * nc, s2); st32(sp, l + 8 * nc, s3);
+  st32_nt(fn.w, q, (s1 + s2) + s3);
   if (hasv) {
     pt = sv1 * pamxS + (dtd * (xyp - xy0)) / dh;
     s1 = pt / papxS;
@@ -249,8 +270,8 @@ __device__ inline void pml_vel_body(const Dims &g, const Coef &c, const Fields &
     s2 = pt / papy;
     pt = sv3 * pamz + (dtd * (yz0 - yzmz)) / dh;
     s3 = pt / papz;
-    s[3 * nc] = s1; s[4 * nc] = s2; s[5 * nc] = s3;
-    st_nt(&fn.v[q], (s1 + s2) + s3);
+    st32(sp, l + 3 * nc, s1); st32(sp, l + 4 * nc, s2); st32(sp, l + 5 * nc, s3);
+    st32_nt(fn.v, q, (s1 + s2) + s3);
   }
 }
 
@@ -385,85 +406,86 @@ __device__ inline void str_fault_body(const Dims &g, const Coef &c, const Fields
 // regions 1 (always) and 2, 3 (free surface only) fused after the update.
 // ---------------------------------------------------------------------------
 __device__ inline void pml_str_body(const Dims &g, const Coef &c, const Fields &fo, const Fields &fn, const Pml &pm, int gbx, int lx) {
-  const int r = pml_region(pm, gbx);
-  const PmlRegion &R = pm.r[r];
-  const int local = (gbx - R.blk0) * kPmlBlock + lx;
-  if (local >= R.ncell) return;
-  const int i2 = local % R.ni + 1;
-  const int t = local / R.ni;
-  const int j2 = t % R.nj + 1;
-  const int k2 = t / R.nj + 1;
-  const int i = R.ib + i2 - 1, j = R.jb + j2 - 1, k = R.kb + k2 - 1;
-  const int q = I3(g, i, j, k), sy = g.sy, sz = g.sz;
-  const float *pr = pm.prof + R.prof_off;
+  PmlCell C;
+  if (!pml_cell(g, pm, gbx, lx, C)) return;
+  const PmlRegion &R = pm.r[C.r];
+  const int q = C.q, sy = g.sy, sz = g.sz, i2 = C.i2, j2 = C.j2, k2 = C.k2, k = C.k;
+  const float *pr = pm.prof;
   const int nI = R.ni + 1, nJ = R.nj + 1, nK = R.nk + 1;
-  const float *amx = pr, *apx = pr + nI, *amxS = pr + 2 * nI, *apxS = pr + 3 * nI;
-  const float *amy = pr + 4 * nI, *apy = amy + nJ, *amyS = amy + 2 * nJ, *apyS = amy + 3 * nJ;
-  const float *amz = pr + 4 * nI + 4 * nJ, *apz = amz + nK, *amzS = amz + 2 * nK, *apzS = amz + 3 * nK;
-  float *s = pm.splits + R.split_off + local;
-  const long long nc = R.ncell;
+  const int amx = R.prof_off, apx = amx + nI, amxS = amx + 2 * nI, apxS = amx + 3 * nI;
+  const int amy = amx + 4 * nI, apy = amy + nJ, amyS = amy + 2 * nJ, apyS = amy + 3 * nJ;
+  const int amz = amx + 4 * nI + 4 * nJ, apz = amz + nK, amzS = amz + 2 * nK, apzS = amz + 3 * nK;
+  float *sp = R.sp;
+  const int nc = R.ncell, l = C.local;
   const float dt = c.dt, dh = c.dh;
+  (void)fo;
   const float *u = fn.u, *v = fn.v, *w = fn.w, *mu = fn.mu;
-  const bool hxy = j <= R.je_v, hxz = j <= R.je_xz;
+  const bool hxy = C.j <= R.je_v, hxz = C.j <= R.je_xz;
   // every load before the first store (see pml_vel_body)
   const float xl = fn.lam[k], xm = mu[k];
-  const float u0 = u[q], up = u[q + 1], v0 = v[q], vmy = v[q - sy], w0 = w[q], wmz = w[q - sz];
-  const float ax = amxS[i2], bx = apxS[i2], ay = amy[j2], by = apy[j2], az = amz[k2], bz = apz[k2];
-  const float sxx1 = s[9 * nc], sxx2 = s[10 * nc], sxx3 = s[11 * nc];
-  const float syy1 = s[12 * nc], syy2 = s[13 * nc], syy3 = s[14 * nc];
-  const float szz1 = s[15 * nc], szz2 = s[16 * nc], szz3 = s[17 * nc];
+  const float u0 = ld32(u, q), up = ld32(u, q + 1), v0 = ld32(v, q), vmy = ld32(v, q - sy), w0 = ld32(w, q),
+              wmz = ld32(w, q - sz);
+  const float ax = ld32(pr, amxS + i2), bx = ld32(pr, apxS + i2), ay = ld32(pr, amy + j2), by = ld32(pr, apy + j2),
+              az = ld32(pr, amz + k2), bz = ld32(pr, apz + k2);
+  const float sxx1 = ld32(sp, l + 9 * nc), sxx2 = ld32(sp, l + 10 * nc), sxx3 = ld32(sp, l + 11 * nc);
+  const float syy1 = ld32(sp, l + 12 * nc), syy2 = ld32(sp, l + 13 * nc), syy3 = ld32(sp, l + 14 * nc);
+  const float szz1 = ld32(sp, l + 15 * nc), szz2 = ld32(sp, l + 16 * nc), szz3 = ld32(sp, l + 17 * nc);
   float mxy = 0, uy = 0, vmx = 0, sxy1 = 0, sxy2 = 0, pamyS = 0, papyS = 0, pamx = 0, papx = 0;
   float mxz = 0, uz = 0, wmx = 0, sxz1 = 0, sxz2 = 0, pamzS = 0, papzS = 0;
   float myz = 0, vz = 0, wy = 0, syz1 = 0, syz2 = 0;
-  if (hxy || hxz) { pamx = amx[i2]; papx = apx[i2]; }
+  if (hxy || hxz) { pamx = ld32(pr, amx + i2); papx = ld32(pr, apx + i2); }
   if (hxy) {
-    mxy = mu[k]; uy = u[q + sy]; vmx = v[q - 1]; sxy1 = s[18 * nc]; sxy2 = s[19 * nc];
-    pamyS = amyS[j2]; papyS = apyS[j2];
-    myz = mu[k + 1]; vz = v[q + sz]; wy = w[q + sy]; syz1 = s[22 * nc]; syz2 = s[23 * nc];
+    mxy = mu[k]; uy = ld32(u, q + sy); vmx = ld32(v, q - 1); sxy1 = ld32(sp, l + 18 * nc); sxy2 = ld32(sp, l + 19 * nc);
+    pamyS = ld32(pr, amyS + j2); papyS = ld32(pr, apyS + j2);
+    myz = mu[k + 1]; vz = ld32(v, q + sz); wy = ld32(w, q + sy); syz1 = ld32(sp, l + 22 * nc);
+    syz2 = ld32(sp, l + 23 * nc);
   }
-  if (hxz) { mxz = mu[k + 1]; uz = u[q + sz]; wmx = w[q - 1]; sxz1 = s[20 * nc]; sxz2 = s[21 * nc]; }
-  if (hxy || hxz) { pamzS = amzS[k2]; papzS = apzS[k2]; }
+  if (hxz) {
+    mxz = mu[k + 1]; uz = ld32(u, q + sz); wmx = ld32(w, q - 1); sxz1 = ld32(sp, l + 20 * nc);
+    sxz2 = ld32(sp, l + 21 * nc);
+  }
+  if (hxy || hxz) { pamzS = ld32(pr, amzS + k2); papzS = ld32(pr, apzS + k2); }
   const float a = xl + 2.0f * xm, b = xl;
   const float diff1 = up - u0, diff2 = v0 - vmy, diff3 = w0 - wmz;
   float pt, s1, s2, s3;
   pt = sxx1 * ax + ((dt * a) * diff1) / dh; s1 = pt / bx;
   pt = sxx2 * ay + ((dt * b) * diff2) / dh; s2 = pt / by;
   pt = sxx3 * az + ((dt * b) * diff3) / dh; s3 = pt / bz;
-  s[9 * nc] = s1; s[10 * nc] = s2; s[11 * nc] = s3;
-  st_nt(&fn.xx[q], (s1 + s2) + s3);
+  st32(sp, l + 9 * nc, s1); st32(sp, l + 10 * nc, s2); st32(sp, l + 11 * nc, s3);
+  st32_nt(fn.xx, q, (s1 + s2) + s3);
   pt = syy1 * ax + ((dt * b) * diff1) / dh; s1 = pt / bx;
   pt = syy2 * ay + ((dt * a) * diff2) / dh; s2 = pt / by;
   pt = syy3 * az + ((dt * b) * diff3) / dh; s3 = pt / bz;
-  s[12 * nc] = s1; s[13 * nc] = s2; s[14 * nc] = s3;
-  st_nt(&fn.yy[q], (s1 + s2) + s3);
+  st32(sp, l + 12 * nc, s1); st32(sp, l + 13 * nc, s2); st32(sp, l + 14 * nc, s3);
+  st32_nt(fn.yy, q, (s1 + s2) + s3);
   pt = szz1 * ax + ((dt * b) * diff1) / dh; s1 = pt / bx;
   pt = szz2 * ay + ((dt * b) * diff2) / dh; s2 = pt / by;
   pt = szz3 * az + ((dt * a) * diff3) / dh; s3 = pt / bz;
-  s[15 * nc] = s1; s[16 * nc] = s2; s[17 * nc] = s3;
-  st_nt(&fn.zz[q], (s1 + s2) + s3);
+  st32(sp, l + 15 * nc, s1); st32(sp, l + 16 * nc, s2); st32(sp, l + 17 * nc, s3);
+  st32_nt(fn.zz, q, (s1 + s2) + s3);
   const bool zero_top = R.ztop && k2 == R.nk;
   if (hxy) {
     const float xmu = (xm + mxy) / 2.0f;
     pt = sxy1 * pamyS + ((dt * xmu) * (uy - u0)) / dh; s1 = pt / papyS;
     pt = sxy2 * pamx + ((dt * xmu) * (v0 - vmx)) / dh; s2 = pt / papx;
-    s[18 * nc] = s1; s[19 * nc] = s2;
-    st_nt(&fn.xy[q], s1 + s2);
+    st32(sp, l + 18 * nc, s1); st32(sp, l + 19 * nc, s2);
+    st32_nt(fn.xy, q, s1 + s2);
   }
   if (hxz) {
     const float xmu = (xm + mxz) / 2.0f;
     pt = sxz1 * pamzS + ((dt * xmu) * (uz - u0)) / dh; s1 = pt / papzS;
     pt = sxz2 * pamx + ((dt * xmu) * (w0 - wmx)) / dh; s2 = pt / papx;
-    st_nt(&fn.xz[q], s1 + s2);
+    st32_nt(fn.xz, q, s1 + s2);
     if (zero_top && j2 <= R.nzx) { s1 = 0.0f; s2 = 0.0f; }
-    s[20 * nc] = s1; s[21 * nc] = s2;
+    st32(sp, l + 20 * nc, s1); st32(sp, l + 21 * nc, s2);
   }
   if (hxy) {
     const float xmu = (xm + myz) / 2.0f;
     pt = syz1 * pamzS + ((dt * xmu) * (vz - v0)) / dh; s1 = pt / papzS;
     pt = syz2 * pamyS + ((dt * xmu) * (wy - w0)) / dh; s2 = pt / papyS;
-    st_nt(&fn.yz[q], s1 + s2);
+    st32_nt(fn.yz, q, s1 + s2);
     if (zero_top && j2 <= R.nzy) { s1 = 0.0f; s2 = 0.0f; }
-    s[22 * nc] = s1; s[23 * nc] = s2;
+    st32(sp, l + 22 * nc, s1); st32(sp, l + 23 * nc, s2);
   }
 }
 
@@ -723,7 +745,7 @@ __device__ inline void rates_body(const Dims &g, const Coef &c, const Fields &fn
     if (t < 256) {
       double acc = 0.0;
       for (long long n = t; n < slab; n += 256) {
-        const int i = g.nabc + 1 + (int)(n % g.nxT), k = g.nabc + 1 + (int)(n / g.nxT);
+        const int k = g.nabc + 1 + (int)(n / g.nxT);
         const float sx = srx[n], sz = srz[n];
         const float m = fn.mu[k];
         acc += (double)((((sqrtf(sx * sx + sz * sz) * m) * c.dh) * c.dh) / per);
