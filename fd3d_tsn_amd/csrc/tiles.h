@@ -140,7 +140,7 @@ __device__ inline void vel_tiled_body(const Dims &g, const Coef &c, const Fields
     }
     const bool adv = j < je && inside;
     float nxy = 0, nyy = 0, nyz = 0;
-    if (adv) { nxy = ld32(xy, q + 2 * sy); nyy = ld32(yy, q + 3 * sy); nyz = ld32(yz, q + 2 * sy); }
+    if (adv) { nxy = ld32(xy, q + 2 * sy); nyy = ld32_nt(yy, q + 3 * sy); nyz = ld32(yz, q + 2 * sy); }
     float uo = 0, vo = 0, wo = 0;
     if (active) { uo = ld32_nt(fo.u, q); vo = ld32_nt(fo.v, q); wo = ld32_nt(fo.w, q); }
     L[0][a0] = xx0; L[1][a0] = xy0; L[2][a0] = xz0; L[3][a0] = yz0; L[4][a0] = zz0;
@@ -247,8 +247,8 @@ __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields
     if (adv) { nv = ld32(v, q + 2 * sy); nu = ld32(u, q + 3 * sy); nw = ld32(w, q + 3 * sy); }
     float oxx = 0, oyy = 0, ozz = 0, oxy = 0, oxz = 0, oyz = 0;
     if (active) {
-      oxx = ld32(fo.xx, q); oyy = ld32(fo.yy, q); ozz = ld32(fo.zz, q);
-      oxy = ld32(fo.xy, q); oxz = ld32(fo.xz, q); oyz = ld32(fo.yz, q);
+      oxx = ld32_nt(fo.xx, q); oyy = ld32_nt(fo.yy, q); ozz = ld32_nt(fo.zz, q);
+      oxy = ld32_nt(fo.xy, q); oxz = ld32_nt(fo.xz, q); oyz = ld32_nt(fo.yz, q);
     }
     L[0][a0] = u0; L[1][a0] = v0; L[2][a0] = w0;
 #pragma unroll
@@ -261,14 +261,14 @@ __device__ inline void str_tiled_body(const Dims &g, const Coef &c, const Fields
       const float dux1 = c1 * (Lu[a0 + 1] - u0), dux2 = c2 * (Lu[a0 + 2] - Lu[a0 - 1]);
       const float dvy1 = c1 * (v0 - vm1), dvy2 = c2 * (vp1 - vm2);
       const float dwz1 = c1 * (w0 - Lw[a0 - W]), dwz2 = c2 * (Lw[a0 + W] - Lw[a0 - 2 * W]);
-      st32(fn.xx, q, (oxx + da * (dux1 + dux2)) + db * (((dvy1 + dvy2) + dwz1) + dwz2));
-      st32(fn.yy, q, (oyy + da * (dvy1 + dvy2)) + db * (((dux1 + dux2) + dwz1) + dwz2));
-      st32(fn.zz, q, (ozz + da * (dwz1 + dwz2)) + db * (((dux1 + dux2) + dvy1) + dvy2));
-      st32(fn.xy, q, oxy + dxy * (((c1 * (up1 - u0) + c2 * (up2 - um1)) + c1 * (v0 - Lv[a0 - 1])) +
+      st32_nt(fn.xx, q, (oxx + da * (dux1 + dux2)) + db * (((dvy1 + dvy2) + dwz1) + dwz2));
+      st32_nt(fn.yy, q, (oyy + da * (dvy1 + dvy2)) + db * (((dux1 + dux2) + dwz1) + dwz2));
+      st32_nt(fn.zz, q, (ozz + da * (dwz1 + dwz2)) + db * (((dux1 + dux2) + dvy1) + dvy2));
+      st32_nt(fn.xy, q, oxy + dxy * (((c1 * (up1 - u0) + c2 * (up2 - um1)) + c1 * (v0 - Lv[a0 - 1])) +
                                   c2 * (Lv[a0 + 1] - Lv[a0 - 2])));
-      st32(fn.xz, q, oxz + dxz * (((c1 * (Lu[a0 + W] - u0) + c2 * (Lu[a0 + 2 * W] - Lu[a0 - W])) +
+      st32_nt(fn.xz, q, oxz + dxz * (((c1 * (Lu[a0 + W] - u0) + c2 * (Lu[a0 + 2 * W] - Lu[a0 - W])) +
                                    c1 * (w0 - Lw[a0 - 1])) + c2 * (Lw[a0 + 1] - Lw[a0 - 2])));
-      st32(fn.yz, q, oyz + dxz * (((c1 * (Lv[a0 + W] - v0) + c2 * (Lv[a0 + 2 * W] - Lv[a0 - W])) +
+      st32_nt(fn.yz, q, oyz + dxz * (((c1 * (Lv[a0 + W] - v0) + c2 * (Lv[a0 + 2 * W] - Lv[a0 - W])) +
                                    c1 * (wp1 - w0)) + c2 * (wp2 - wm1)));
     }
     if (adv) {
