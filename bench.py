@@ -92,6 +92,25 @@ def profiled_traffic(workload_name, kernel):
     return None, None
 
 
+STREAM_MIX = {"vel_interior": "velocity_sweep_mix", "str_interior": "stress_sweep_mix",
+              "pml_vel": "pml_velocity_mix", "pml_str": "pml_stress_mix"}
+
+
+def stream_ceiling(kernel):
+    """Measured HBM rate of a stencil-free kernel with this sweep's words
+    read and written per cell (profiles/hbm_stream_ceilings.md, made by
+    fd3d_tsn_amd/lib/hbm_stream); None when absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_stream_ceilings.md")
+    if kernel not in STREAM_MIX or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) == 3 and w[0] == "stream_ceiling_GBs" and w[1] == STREAM_MIX[kernel]:
+                return float(w[2])
+    return None
+
+
 def cpu_baseline(case, steps):
     """Reference CPU build (oracle/_ref, flang -O3 -ffast-math, serial) on the
     same grid for a few steps; falls back to the oracle restatement."""
@@ -303,7 +322,20 @@ def main():
             "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg}
     if traffic_src:
         roof["traffic_source"] = traffic_src
-    kernels = {k: {"avg_ms": v[0] / v[1], "launches": v[1]} for k, v in sorted(kt.items())}
+    ceil = stream_ceiling(name)
+    if ceil and alg:
+        roof["stream_ceiling"] = {"GBs": ceil, "frac": alg / (avg_ms * 1e-3) / 1e9 / ceil,
+                                  "source": "profiles/hbm_stream_ceilings.md"}
+    kernels = {}
+    for k, v in sorted(kt.items()):
+        kernels[k] = {"avg_ms": v[0] / v[1], "launches": v[1]}
+        ncell = cells_of(case, k) if k in ("vel_interior", "str_interior") else (npml if k.startswith("pml") else None)
+        if k in BYTES_PER_CELL and ncell:
+            gbs = BYTES_PER_CELL[k] * ncell / (v[0] / v[1] * 1e-3) / 1e9
+            kernels[k]["alg_GBs"] = gbs
+            c = stream_ceiling(k)
+            if c:
+                kernels[k]["of_stream_ceiling"] = gbs / c
     out = {
         "metric": "Gcell-updates/s", "value": value, "unit": "Gcell-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,

@@ -1,0 +1,97 @@
+// HBM stream-mix probe: the bandwidth a kernel with the sweeps' access mix
+// can reach when it does no stencil work.  Each mix reads NR and writes NW
+// float arrays of the large slip-weakening grid's padded shape (x pitch
+// 1088, 532 x 552 rows), one element per lane, marching 64 x 16 x-z tiles
+// along y over 32 rows with the sweeps' XCD-aware block order, so the
+// result is the ceiling of a sweep that moves the same words per cell.
+// usage: hbm_stream [repetitions]
+// Prints one JSON line per mix: bytes moved per launch, average ms, GB/s.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+namespace {
+
+constexpr int NX = 1024, NY = 512, NZ = 512;   // cells marched (the interior of the grid)
+constexpr int PX = 1088, NYT = 532, NZT = 552;  // array shape
+constexpr int ROWS = 32;
+
+__device__ inline int xcd_order(int b, int n) {
+  const int q = n >> 3, r = n & 7, xcd = b & 7, loc = b >> 3;
+  return xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+}
+
+template <int NR, int NW>
+__global__ void __launch_bounds__(1024) mix(float *const *in, float *const *out) {
+  const int gx = NX / 64, gz = NZ / 16;
+  const int b = xcd_order(blockIdx.x, gridDim.x);
+  const int bx = b % gx, bz = (b / gx) % gz, by = b / (gx * gz);
+  const int x = bx * 64 + threadIdx.x, z = bz * 16 + threadIdx.y;
+  const long long sy = PX, sz = (long long)PX * NYT;
+  for (int j = by * ROWS; j < by * ROWS + ROWS && j < NY; ++j) {
+    const long long q = x + j * sy + z * sz;
+    float v[NR > 0 ? NR : 1];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) v[r] = in[r][q];
+    float s = 0.0f;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s += v[r];
+    if (NW == 0 && s == -1.2345e30f) in[0][q] = 0.0f;   // keeps read-only loads alive
+#pragma unroll
+    for (int w = 0; w < NW; ++w) __builtin_nontemporal_store(s * (float)(w + 1), &out[w][q]);
+  }
+}
+
+template <int NR, int NW>
+void run(const char *name, std::vector<float *> &arr, int reps) {
+  std::vector<float *> in(arr.begin(), arr.begin() + NR), out(arr.begin() + NR, arr.begin() + NR + NW);
+  float **din = nullptr, **dout = nullptr;
+  (void)hipMalloc(&din, sizeof(float *) * (NR + 1));
+  (void)hipMalloc(&dout, sizeof(float *) * (NW + 1));
+  if (NR) (void)hipMemcpy(din, in.data(), sizeof(float *) * NR, hipMemcpyHostToDevice);
+  if (NW) (void)hipMemcpy(dout, out.data(), sizeof(float *) * NW, hipMemcpyHostToDevice);
+  const int blocks = (NX / 64) * (NZ / 16) * ((NY + ROWS - 1) / ROWS);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL((mix<NR, NW>), dim3(blocks), dim3(64, 16), 0, 0, din, dout);
+  (void)hipDeviceSynchronize();
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL((mix<NR, NW>), dim3(blocks), dim3(64, 16), 0, 0, din, dout);
+  (void)hipEventRecord(e1, 0);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  const double bytes = 4.0 * (NR + NW) * (double)NX * NY * NZ;
+  printf("{\"mix\": \"%s\", \"read_words\": %d, \"written_words\": %d, \"bytes_per_launch\": %.0f, "
+         "\"avg_ms\": %.4f, \"GBs\": %.1f}\n",
+         name, NR, NW, bytes, ms / reps, bytes / (ms / reps * 1e-3) / 1e9);
+  fflush(stdout);
+  (void)hipFree(din);
+  (void)hipFree(dout);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  const int reps = argc > 1 ? atoi(argv[1]) : 5;
+  const size_t n = (size_t)PX * NYT * NZT;
+  std::vector<float *> arr(39);
+  for (auto &p : arr) {
+    if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) {
+      fprintf(stderr, "hipMalloc failed\n");
+      return 1;
+    }
+    (void)hipMemset(p, 0, n * sizeof(float));
+  }
+  run<1, 1>("copy", arr, reps);
+  run<8, 0>("read_only", arr, reps);
+  run<9, 3>("velocity_sweep_mix", arr, reps);
+  run<9, 6>("stress_sweep_mix", arr, reps);
+  run<15, 12>("pml_velocity_mix", arr, reps);
+  run<18, 21>("pml_stress_mix", arr, reps);
+  for (auto p : arr) (void)hipFree(p);
+  return 0;
+}
