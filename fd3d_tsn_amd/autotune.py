@@ -88,7 +88,7 @@ def candidates(case: C.Case) -> List[Tuple[bool, int, int]]:
 def tune(solver: "capi.Solver", steps: int = 3, warmup: int = 1, save: bool = True) -> Dict:
     """Time every candidate shape on a live problem (its state advances by
     the steps taken; results do not depend on the shape).  Returns and
-    caches {"vel": {"tz x ny": ms}, "str": {...}, "pml": {"flat": ms, "zmarch": ms}}."""
+    caches {"vel": {"tz x ny": ms}, "str": {...}, "pml": {"flat": ms, "tiled": ms}}."""
     case = solver.case
     before = solver.get_tuning()
     table = {"vel": {}, "str": {}, "pml": {}}
@@ -105,7 +105,7 @@ def tune(solver: "capi.Solver", steps: int = 3, warmup: int = 1, save: bool = Tr
                 table["str"][key(wide, tz, ny)] = kt["str_interior"][0] / kt["str_interior"][1]
             if "pml_vel" in kt:
                 pm = (kt["pml_vel"][0] / kt["pml_vel"][1]) + (kt["pml_str"][0] / kt["pml_str"][1])
-                table["pml"]["zmarch" if pml_mode & 2 else "flat"] = pm
+                table["pml"]["tiled" if pml_mode & 2 else "flat"] = pm
     solver.set_tuning(*before)
     if save:
         db = _load()
@@ -141,7 +141,7 @@ def choose(case: C.Case, tz: Optional[int] = None) -> Optional[Tuple[int, int, i
     if tv is None or ts is None:
         return None
     mode = 1 | (8 if wv else 0) | (16 if ws else 0)
-    if e.get("pml") and e["pml"].get("zmarch", 1e30) < e["pml"].get("flat", 1e30):
+    if e.get("pml") and e["pml"].get("tiled", 1e30) < e["pml"].get("flat", 1e30):
         mode = 3
     return (mode, tv, nv, ts, ns)
 

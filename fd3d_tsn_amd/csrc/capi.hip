@@ -94,7 +94,7 @@ int build_regions(const fd3d_config &c, const Dims &g, Pml &pm, long long &nspli
   pm.nreg = (int)sp.size();
   nsplit = 0;
   nprof = 0;
-  int blk = 0, blkz = 0;
+  int blk = 0;
   for (int r = 0; r < pm.nreg; ++r) {
     PmlRegion &R = pm.r[r];
     const Spec &s = sp[r];
@@ -109,18 +109,44 @@ int build_regions(const fd3d_config &c, const Dims &g, Pml &pm, long long &nspli
     R.ztop = s.ztop; R.nzx = s.nzx; R.nzy = s.nzy;
     R.blk0 = blk;
     blk += (R.ncell + kPmlBlock - 1) / kPmlBlock;
-    R.nxyb = (R.ni * R.nj + kPmlBlock - 1) / kPmlBlock;
-    R.nzc = (R.nk + kPmlZChunk - 1) / kPmlZChunk;
-    R.blkz0 = blkz;
-    blkz += R.nxyb * R.nzc;
     R.split_off = nsplit;
     nsplit += (long long)kSplits * R.ncell;
     R.prof_off = (int)nprof;
     nprof += 4LL * (R.ni + 1) + 4LL * (R.nj + 1) + 4LL * (R.nk + 1);
   }
   pm.nblk = blk;
-  pm.nblkz = blkz;
   return 0;
+}
+
+// Blocks of the tiled PML sweeps (pml.hip): 64 x kPmlTZ x-z tiles on the
+// interior tile grid (origins i = nabc+1 + 64 bx, k = nabc+1 + kPmlTZ bz)
+// that hold PML cells, each with the rows of the regions it meets, cut
+// into runs of at most kPmlRows rows: (bx, bz, j0, j1).
+std::vector<int4> build_pml_tiles(const Dims &g, const Pml &pm) {
+  std::vector<int4> t;
+  const int o = g.nabc + 1;
+  auto fdiv = [](int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); };
+  int ilo = g.nxt, ihi = 0, klo = g.nzt, khi = 0;
+  for (int r = 0; r < pm.nreg; ++r) {
+    ilo = std::min(ilo, pm.r[r].ib); ihi = std::max(ihi, pm.r[r].ie);
+    klo = std::min(klo, pm.r[r].kb); khi = std::max(khi, pm.r[r].ke);
+  }
+  for (int bz = fdiv(klo - o, kPmlTZ); bz <= fdiv(khi - o, kPmlTZ); ++bz) {
+    const int k0 = o + kPmlTZ * bz, k1 = k0 + kPmlTZ - 1;
+    for (int bx = fdiv(ilo - o, 64); bx <= fdiv(ihi - o, 64); ++bx) {
+      const int i0 = o + 64 * bx, i1 = i0 + 63;
+      int jlo = g.nyt + 1, jhi = 0;
+      for (int r = 0; r < pm.nreg; ++r) {
+        const PmlRegion &R = pm.r[r];
+        if (std::max(i0, R.ib) <= std::min(i1, R.ie) && std::max(k0, R.kb) <= std::min(k1, R.ke)) {
+          jlo = std::min(jlo, R.jb);
+          jhi = std::max(jhi, R.je_uw);
+        }
+      }
+      for (int j = jlo; j <= jhi; j += kPmlRows) t.push_back(make_int4(bx, bz, j, std::min(jhi, j + kPmlRows - 1)));
+    }
+  }
+  return t;
 }
 
 // init_pml damping profiles (fd3d_deriv.f90:1439-1491), stored as (1-w) and
@@ -390,6 +416,18 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
     if ((rc = dalloc(h, pp, n2))) return bail(rc);
   if ((rc = dalloc(h, &p.pml.splits, (size_t)nsplit))) return bail(rc);
   for (int r = 0; r < p.pml.nreg; ++r) p.pml.r[r].sp = p.pml.splits + p.pml.r[r].split_off;
+  {
+    const std::vector<int4> tiles = build_pml_tiles(p.g, p.pml);
+    p.pml.ntiles = (int)tiles.size();
+    if ((rc = dalloc(h, &p.pml.tiles, std::max<size_t>(tiles.size(), 1)))) return bail(rc);
+    if (!tiles.empty() && hipMemcpy(p.pml.tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice) !=
+                              hipSuccess)
+      return bail(FD3D_E_HIP);
+    PmlRegion *rd = nullptr;
+    if ((rc = dalloc(h, &rd, (size_t)kMaxRegions))) return bail(rc);
+    if (hipMemcpy(rd, p.pml.r, sizeof(p.pml.r), hipMemcpyHostToDevice) != hipSuccess) return bail(FD3D_E_HIP);
+    p.pml.rdev = rd;
+  }
   if ((rc = dalloc(h, &p.pml.prof, (size_t)nprof))) return bail(rc);
   const size_t slab = (size_t)p.g.nxT * p.g.nzT;
   if ((rc = dalloc(h, &p.out.ring, (size_t)h->cfg.ring_steps * FD3D_NOUT * slab))) return bail(rc);

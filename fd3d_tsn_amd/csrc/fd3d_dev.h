@@ -101,7 +101,6 @@ struct PmlRegion {
   float *sp;               // the region's 24 splits, split n at sp + n * ncell
   int ztop, nzx, nzy;      // zero xz/yz splits at k2 = nk for j2 <= nzx / nzy
   int blk0;                // first block of this region in the fused PML launch
-  int blkz0, nxyb, nzc;    // z-march launch: first block, (i,j) blocks, z chunks
   long long split_off;     // offset of split 0 in the pool (floats)
   int prof_off;            // offset of the profile block in the profile pool
 };
@@ -126,15 +125,23 @@ __device__ __forceinline__ float ld_nt(const float *p) { return __builtin_nontem
 constexpr int kMaxRegions = 5;
 constexpr int kSplits = 24;
 constexpr int kPmlBlock = 256;
-constexpr int kPmlZChunk = 64;   // planes per thread in the z-march PML sweeps
+// tiled PML sweeps (pml.hip): 64 x kPmlTZ x-z tiles on the interior tile
+// grid, each block marching at most kPmlRows rows along y
+constexpr int kPmlTZ = 8;
+constexpr int kPmlRows = 32;
 
 struct Pml {
-  int nreg, nblk, nblkz;
+  int nreg, nblk;
   PmlRegion r[kMaxRegions];
   float *splits;
   // per region 12 profile arrays: for p in (ox, oxS, oy, oyS, oz, ozS):
   // (1 - omega) then (1 + omega), 1-based, lengths ni+1 / nj+1 / nk+1
   float *prof;
+  // tiled sweeps: one (bx, bz, j0, j1) entry per block, and a device copy
+  // of the region table (indexed by a wave-uniform region number)
+  int4 *tiles;
+  int ntiles;
+  const PmlRegion *rdev;
 };
 
 struct Ctl {
@@ -201,8 +208,8 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s);
 hipError_t launch_interior(bool stress, const Problem &p, hipStream_t s);
 // Fused interior velocity + stress sweep, fused.hip.
 hipError_t launch_fused(const Problem &p, hipStream_t s);
-// z-march PML sweep (velocity or stress), pml.hip.
-hipError_t launch_pml_zm(bool stress, const Problem &p, hipStream_t s);
+// Tiled PML sweep (velocity or stress), pml.hip.
+hipError_t launch_pml_tiled(bool stress, const Problem &p, hipStream_t s);
 // Fill helpers used by the prologue.
 hipError_t fill_f32(float *p, float v, long long n, hipStream_t s);
 

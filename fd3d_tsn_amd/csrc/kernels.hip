@@ -140,7 +140,7 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
       break;
     case K_PML_VEL:
       if (p.pml.nblk == 0) return hipSuccess;
-      if (p.tune.mode & 2) return launch_pml_zm(false, p, s);
+      if (p.tune.mode & 2) return launch_pml_tiled(false, p, s);
       hipLaunchKernelGGL(k_pml_vel, dim3(p.pml.nblk), dim3(kPmlBlock), 0, s, g, p.c, p.f[0], p.f[1], p.pml, p.ctl);
       break;
     case K_FUVW_UV:
@@ -160,7 +160,7 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
       break;
     case K_PML_STR:
       if (p.pml.nblk == 0) return hipSuccess;
-      if (p.tune.mode & 2) return launch_pml_zm(true, p, s);
+      if (p.tune.mode & 2) return launch_pml_tiled(true, p, s);
       hipLaunchKernelGGL(k_pml_str, dim3(p.pml.nblk), dim3(kPmlBlock), 0, s, g, p.c, p.f[0], p.f[1], p.pml, p.ctl);
       break;
     case K_FRES:
