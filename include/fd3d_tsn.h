@@ -176,7 +176,7 @@ int fd3d_download_source_bins(fd3d_handle *h, float *msrx, float *msrz, float *m
 /* Launch shapes of the 3D sweeps.  mode bit 0: interior sweeps where a
  * block owns a 64 x tz tile of the x-z plane and marches ny rows along y
  * (tz in {4, 8, 16}, ny >= 1), else one thread per cell; mode bit 1: PML
- * sweeps marching z in register windows, else one thread per cell; mode
+ * sweeps as 64 x 8 x-z tiles marching y, else one thread per cell; mode
  * bit 2: one fused velocity + stress pass over the interior (full space
  * only, ny_vel rows per block); mode bits 3 and 4 (with bit 0): velocity
  * and stress tiles 128 wide instead of 64 (tz in {4, 8}).  fd3d_create picks a default from
