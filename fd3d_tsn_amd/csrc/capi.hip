@@ -314,7 +314,15 @@ int check_ctl(fd3d_handle *h) { return fd3d_handle_check_ctl(h, h->stream); }
 // kernels read only the old wavefield set and the stress kernels only the
 // new velocities and old stresses, so vel_fault and pml_vel may run before
 // the fused interior sweep, which needs their new velocities.
+void drop_graph(fd3d_handle *h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+}
+
 void build_seq(fd3d_handle *h) {
+  drop_graph(h);
   const Problem &p = h->p;
   const bool fused = (p.tune.mode & 4) && p.g.fspace;
   h->seq.clear();
@@ -636,6 +644,25 @@ int fd3d_init_state(fd3d_handle *h) {
   return 0;
 }
 
+// One step of h->seq captured into a graph on h->stream.  Kernel arguments
+// are fixed for the life of the handle (the wavefield set and the time
+// come from the device step counter), so one graph serves every step.
+static int capture_step(fd3d_handle *h) {
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  for (KernelId id : h->seq) {
+    const hipError_t e = launch(id, h->p, h->stream);
+    if (e != hipSuccess) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(h->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return fail(h, FD3D_E_HIP, std::string("capture ") + kKernelNames[id] + ": " + hipGetErrorString(e));
+    }
+  }
+  HIPCHK(h, hipStreamEndCapture(h->stream, &h->graph));
+  HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
 int fd3d_step(fd3d_handle *h, int nsteps, float *out, float *seis) {
   if (!h) return FD3D_E_INVALID;
   if (!h->inited) return fail(h, FD3D_E_STATE, "fd3d_init_state first");
@@ -648,11 +675,20 @@ int fd3d_step(fd3d_handle *h, int nsteps, float *out, float *seis) {
   while (done < nsteps) {
     const int pos = h->steps % ring;
     const int chunk = std::min(nsteps - done, ring - pos);
-    for (int s = 0; s < chunk; ++s) {
-      for (KernelId id : h->seq) {
-        int rc = launch_one(h, id);
+    if (h->prof) {
+      // kernel by kernel, with events around each launch
+      for (int s = 0; s < chunk; ++s) {
+        for (KernelId id : h->seq) {
+          int rc = launch_one(h, id);
+          if (rc) return rc;
+        }
+      }
+    } else {
+      if (!h->gexec) {
+        int rc = capture_step(h);
         if (rc) return rc;
       }
+      for (int s = 0; s < chunk; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
     }
     if (out) {
       HIPCHK(h, hipMemcpyAsync(out + (size_t)done * FD3D_NOUT * slab, h->p.out.ring + (size_t)pos * FD3D_NOUT * slab,
@@ -862,6 +898,7 @@ void fd3d_destroy(fd3d_handle *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (auto &e : h->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto e : h->pool) (void)hipEventDestroy(e);
+  drop_graph(h);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

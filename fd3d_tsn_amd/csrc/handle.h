@@ -33,8 +33,11 @@ struct fd3d_handle {
   std::vector<hipEvent_t> pool;
   double kms[K_COUNT] = {0};
   long kcount[K_COUNT] = {0};
-  // step sequence
+  // step sequence, and one step of it captured in a HIP graph (replayed by
+  // fd3d_step when profiling is off; rebuilt when the sequence changes)
   std::vector<KernelId> seq;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   // the batch this problem is advanced in (fd3d_batch_add), or null
   fd3d_batch *batch = nullptr;
 };
