@@ -38,6 +38,10 @@ struct fd3d_handle {
   std::vector<KernelId> seq;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // second stream and fork/join events of the captured step: the kernels of
+  // a half-step that write disjoint cells run side by side
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   // the batch this problem is advanced in (fd3d_batch_add), or null
   fd3d_batch *batch = nullptr;
 };
