@@ -37,11 +37,13 @@ BYTES_PER_CELL = {
     "str_interior": 15 * 4,   # reads u, v, w + 6 stresses; writes 6 stresses
     "pml_vel": 27 * 4,        # 9 splits read+write, 6 stresses read, 3 velocities written
     "pml_str": 39 * 4,        # 15 splits read+write, 3 velocities read, 6 stresses written
+    "fused_interior": 18 * 4,  # reads 6 stresses + u, v, w; writes u, v, w + 6 stresses
 }
-# The same per step: 108 B per bulk cell, 264 B per PML cell.  The survey's
-# convention counts the medium as three streamed 3D arrays (120 B / 276 B);
-# the JSON line reports the HBM fraction on both.
-MOVED_BULK, MOVED_PML = 108.0, 264.0
+# The same per step: 108 B per bulk cell with the separate sweeps (72 B with
+# the fused sweep), 264 B per PML cell.  The survey's convention counts the
+# two separate sweeps with the medium as three streamed 3D arrays (120 B /
+# 276 B); the JSON line reports the HBM fraction on both.
+MOVED_BULK, MOVED_BULK_FUSED, MOVED_PML = 108.0, 72.0, 264.0
 
 
 def parse():
@@ -75,7 +77,7 @@ def cells_of(case, kernel):
     nxi = case.nxt - 2 * case.nabc
     nyi = case.nyt - 2 - case.nabc
     nzi = case.nzt - case.nfs - case.nabc
-    if kernel in ("vel_interior", "str_interior"):
+    if kernel in ("vel_interior", "str_interior", "fused_interior"):
         return nxi * nyi * nzi
     return None
 
@@ -96,7 +98,7 @@ def profiled_traffic(workload_name, kernel):
 
 
 STREAM_MIX = {"vel_interior": "velocity_sweep_mix", "str_interior": "stress_sweep_mix",
-              "pml_vel": "pml_velocity_mix", "pml_str": "pml_stress_mix"}
+              "fused_interior": "fused_sweep_mix", "pml_vel": "pml_velocity_mix", "pml_str": "pml_stress_mix"}
 
 
 def stream_ceiling(kernel):
@@ -309,8 +311,9 @@ def main():
     kt = s.kernel_times()
     s.set_profiling(False)
     step_bytes, nbulk, npml = s.step_bytes()
-    moved = MOVED_BULK * nbulk + MOVED_PML * npml
     tuning = s.get_tuning()
+    fused = bool(tuning[0] & 4) and case.fspace
+    moved = (MOVED_BULK_FUSED if fused else MOVED_BULK) * nbulk + MOVED_PML * npml
     s.close()
 
     ms_per_step = elapsed * 1e3 / a.steps
@@ -319,7 +322,7 @@ def main():
     dom = max(kt.items(), key=lambda kv: kv[1][0])
     name, (ms_tot, launches) = dom
     avg_ms = ms_tot / launches
-    ncell = cells_of(case, name)
+    ncell = npml if name.startswith("pml") else cells_of(case, name)
     if name in BYTES_PER_CELL and ncell:
         alg = BYTES_PER_CELL[name] * ncell
     else:
@@ -340,7 +343,7 @@ def main():
     kernels = {}
     for k, v in sorted(kt.items()):
         kernels[k] = {"avg_ms": v[0] / v[1], "launches": v[1]}
-        ncell = cells_of(case, k) if k in ("vel_interior", "str_interior") else (npml if k.startswith("pml") else None)
+        ncell = npml if k.startswith("pml") else cells_of(case, k)
         if k in BYTES_PER_CELL and ncell:
             gbs = BYTES_PER_CELL[k] * ncell / (v[0] / v[1] * 1e-3) / 1e9
             kernels[k]["alg_GBs"] = gbs
@@ -361,8 +364,9 @@ def main():
                      "frac": moved / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "convention_bytes_per_step": step_bytes,
                      "convention_frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "note": "alg bytes: 108 B/bulk cell, 264 B/PML cell (layered medium, no medium "
-                             "arrays streamed); convention: 120/276 B with the medium as three 3D arrays"},
+                     "note": f"alg bytes: {72 if fused else 108} B/bulk cell ({'fused' if fused else 'separate'} "
+                             "interior sweep), 264 B/PML cell (layered medium, no medium arrays streamed); "
+                             "convention: 120/276 B, separate sweeps with the medium as three 3D arrays"},
         "roofline": roof,
         "kernels": kernels,
     }

@@ -15,6 +15,10 @@ Measured tables are cached per device and size class in a JSON file
 size class rounds each interior extent to the nearest of a fixed ladder
 of sizes and keeps the PML width and the free-surface setting.
 
+For full-space problems the fused velocity + stress sweep (mode bit 2)
+is timed too, at several rows per block; :func:`choose` takes it when it
+beats the best pair of separate sweeps.
+
 * :func:`tune` measures one problem and stores its table;
 * :func:`choose` returns the best (tz, ny) of a problem's class, or the
   best ny for a given tz (the batch engine uses one tz for all members);
@@ -37,6 +41,7 @@ CACHE = os.environ.get("FD3D_TUNING_CACHE", os.path.join(HERE, "tuning_cache.jso
 LADDER = (32, 48, 64, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024, 1536, 2048)
 TZS = (4, 8, 16)
 NYS = (8, 16, 32, 64, 128)
+FUSED_NYS = (32, 64, 128, 256, 512)
 
 
 def size_class(case: C.Case) -> str:
@@ -88,10 +93,24 @@ def candidates(case: C.Case) -> List[Tuple[bool, int, int]]:
 def tune(solver: "capi.Solver", steps: int = 3, warmup: int = 1, save: bool = True) -> Dict:
     """Time every candidate shape on a live problem (its state advances by
     the steps taken; results do not depend on the shape).  Returns and
-    caches {"vel": {"tz x ny": ms}, "str": {...}, "pml": {"flat": ms, "tiled": ms}}."""
+    caches {"vel": {"tz x ny": ms}, "str": {...}, "pml": {"flat": ms, "tiled": ms}}
+    and, for full-space problems, "fused": {"ny": ms}."""
     case = solver.case
     before = solver.get_tuning()
     table = {"vel": {}, "str": {}, "pml": {}}
+    if case.fspace:
+        nyi = case.nyt - 2 - case.nabc
+        table["fused"] = {}
+        for ny in FUSED_NYS:
+            if ny > max(32, 2 * nyi):
+                continue
+            solver.set_tuning(5, 16, ny, 16, ny)
+            solver.step(warmup, outputs=False)
+            solver.set_profiling(True)
+            solver.step(steps, outputs=False)
+            kt = solver.kernel_times()
+            solver.set_profiling(False)
+            table["fused"][str(ny)] = kt["fused_interior"][0] / kt["fused_interior"][1]
     for wide, tz, ny in candidates(case):
         for pml_mode in ((1, 3) if (wide, tz, ny) == (False, 16, 32) else (1,)):
             solver.set_tuning(pml_mode | (24 if wide else 0), tz, ny, tz, ny)
@@ -143,6 +162,13 @@ def choose(case: C.Case, tz: Optional[int] = None) -> Optional[Tuple[int, int, i
     mode = 1 | (8 if wv else 0) | (16 if ws else 0)
     if e.get("pml") and e["pml"].get("tiled", 1e30) < e["pml"].get("flat", 1e30):
         mode = 3
+    # the fused sweep (single problems only: the batch engine runs the
+    # separate sweeps) when it beats the best separate pair
+    fz = e.get("fused")
+    if tz is None and fz and case.fspace:
+        fny, fms = min(((int(k), v) for k, v in fz.items()), key=lambda kv: kv[1])
+        if fms < _best(e["vel"])[3] + _best(e["str"])[3]:
+            return (mode | 4, tv, fny, ts, ns)
     return (mode, tv, nv, ts, ns)
 
 
@@ -176,7 +202,7 @@ def batch_plan(cases: Iterable[C.Case]) -> Tuple[int, List[Optional[Tuple[int, i
 
 
 def main():
-    """python -m fd3d_tsn_amd.autotune [batched|served N|large] : tune the
+    """python -m fd3d_tsn_amd.autotune [batched|served N|single|large] : tune the
     size classes of a workload on the GPU and update the cache."""
     import sys
     import time
@@ -187,6 +213,8 @@ def main():
     elif which == "served":
         from . import serve
         cs = [r.case for r in serve.c5_requests(2000)[:n]]
+    elif which == "single":
+        cs = [C.c1_plain_sw(), C.c2_large_sw(), C.c3_fvw()]
     else:
         cs = [C.c2_large_sw()]
     seen = set()
@@ -199,7 +227,9 @@ def main():
         with capi.Solver(c.replace(ntfd=8, stations=[])) as s:
             tab = tune(s)
         bv, bs = _best(tab["vel"]), _best(tab["str"])
-        print(f"[autotune] {k}: vel {key(*bv[:3])} {bv[3]:.3f} ms, str {key(*bs[:3])} {bs[3]:.3f} ms "
+        fz = tab.get("fused") or {}
+        fus = (" fused " + " ".join(f"{ny}:{ms:.3f}" for ny, ms in fz.items())) if fz else ""
+        print(f"[autotune] {k}: vel {key(*bv[:3])} {bv[3]:.3f} ms, str {key(*bs[:3])} {bs[3]:.3f} ms{fus} "
               f"({time.perf_counter()-t0:.1f} s)", flush=True)
 
 

@@ -9,25 +9,29 @@
 // once: 18 words.  The two wavefield sets make this race-free: every block
 // reads the old set and writes the new one.
 //
-// Work of a block (64 x 16 lanes).  Lane (tx, ty) owns position
-// x = tx - 2, z = ty - 2 of an x-z tile whose core, the cells that get new
-// stresses, is x in [0, 59], z in [0, 11]; the block marches along y over
-// the rows [j0, je] of its chunk, from j0 - 2 to je + 2.  Iteration r:
+// Work of a block (16 wavefronts of 64 lanes).  The core, the cells that
+// get new stresses, is x in [0, 63], z in [0, 10] of an x-z tile: its x
+// extent is one 256-byte row segment of every array, so every load and
+// store of a core row moves whole cache lines.  Wavefront w < 15 owns the
+// positions x = lane, z = w - 2 (the core rows and a 2-row ring above and
+// below); wavefront 15 owns the x-ring, x in {-2, -1, 64, 65} for every
+// z in [-2, 12].  The block marches along y over the rows [j0, je] of its
+// chunk, from j0 - 2 to je + 2.  Iteration r:
 //   1. the old-stress planes of row r (loaded into registers during
-//      iteration r - 1) go into one of two LDS buffers, and every lane puts
+//      iteration r - 2) go into one of two LDS buffers, and every lane puts
 //      its new velocity of row r - 2 (from its register window) into one of
 //      two velocity planes; one barrier;
-//   2. the loads of row r + 1 are issued; every lane forms the new velocity
+//   2. the loads of row r + 2 are issued; every lane forms the new velocity
 //      of its position at row r from the stress planes (x and z neighbours)
 //      and its register windows of xy, yy, yz (y neighbours); core lanes
 //      form the new stresses of row r - 2 from the velocity plane (x and z
 //      neighbours) and their velocity windows (y neighbours).
-// Lanes on the 2-cell ring around the core recompute velocities that the
-// neighbouring tiles own, so no block waits on another, and every lane
-// does the same work.  Positions outside the interior (PML cells, the
-// near-fault rows j >= nyt-1, the outer faces) take the values pml_vel and
-// vel_fault already wrote into the new set: the kernel runs after them and
-// before str_fault and pml_str, which read its new velocities.
+// Lanes on the ring around the core recompute velocities that the
+// neighbouring tiles own, so no block waits on another.  Positions outside
+// the interior (PML cells, the near-fault rows j >= nyt-1, the outer faces)
+// take the values pml_vel and vel_fault already wrote into the new set: the
+// kernel runs after them and before str_fault and pml_str, which read its
+// new velocities.
 //
 // Per-point arithmetic is that of tiles.h (the same operands, the same
 // association), so the result is bit-identical to the separate sweeps.
@@ -38,74 +42,99 @@
 namespace fd3d {
 namespace {
 
-// global-memory floats: keeps plane loads global_load when their field
-// pointer is chosen per wavefront
-typedef __attribute__((address_space(1))) const float gfloat;
+constexpr int LX = 64;             // core x extent: one wavefront, one 256-byte row segment
+constexpr int NWV = 16;            // wavefronts per block
+constexpr int ZC = NWV - 5;        // core rows along z (11)
+constexpr int NZP = ZC + 4;        // position rows, z in [-2, ZC + 1]
+constexpr int NT = LX * NWV;
+constexpr int PW = 72;             // LDS plane row: x in [-4, 67]
+constexpr int PX0 = 4;             // its column of x = 0
 
-constexpr int LX = 64, LZ = 16;          // lanes along x and z
-constexpr int CX = LX - 4, CZ = LZ - 4;  // core: 60 x 12 cells
-constexpr int NT = LX * LZ;
+// Old-stress planes of one row.  Plane f covers the rows z in [z0, z0 +
+// rows) that the velocity stencils of the positions reach, at LDS rows
+// base .. base + rows - 1 (PW floats each), and the columns x in [-left,
+// 63 + right]:
+//   xx: z in [-2, 12], x in [-4, 66]      xy: z in [-2, 12], x in [-3, 67]
+//   xz: z in [-4, 13], x in [-3, 67]      yz: z in [-4, 13], x in [-2, 65]
+//   zz: z in [-3, 14], x in [-2, 65]
+// A plane row is 18 aligned float4 chunks (x = -4 + 4q, q = 0..17; the
+// core origin starts a 256-byte boundary), loaded one chunk per lane with
+// 16-byte loads, field by field: a field of R rows takes ceil(18 R / 64)
+// wavefront loads.
+constexpr int PROWS_XY = NZP, PROWS_Z = ZC + 7;
+constexpr int PBASE_XY = NZP, PBASE_XZ = 2 * NZP, PBASE_YZ = PBASE_XZ + PROWS_Z, PBASE_ZZ = PBASE_YZ + PROWS_Z;
+constexpr int NPR = PBASE_ZZ + PROWS_Z;        // plane rows (84)
+constexpr int PCH = PW / 4;                     // chunks per plane row
+constexpr int NLXY = (PROWS_XY * PCH + 63) / 64, NLZ = (PROWS_Z * PCH + 63) / 64;
+constexpr int M_XY = NLXY, M_XZ = 2 * NLXY, M_YZ = M_XZ + NLZ, M_ZZ = M_YZ + NLZ;
+constexpr int NWL = M_ZZ + NLZ;                 // wavefront loads per row (28)
+constexpr int NPF = (NWL + NWV - 1) / NWV;      // per wavefront (2)
+static_assert(4 * NZP <= 64, "the x-ring fits one wavefront");
 
-// Old-stress planes of one row, relative to the core origin; every plane
-// spans what the velocity stencils of the lane positions x in [-2, 61],
-// z in [-2, 13] read.  The five planes lie back to back in one buffer,
-// each padded to whole wavefronts (64 cells) so that a wavefront's loads
-// of a plane row all come from one field.
-template <int X0, int Z0, int W_, int H_, int OFF>
-struct SP {
-  static constexpr int W = W_, N = W_ * H_, NW = (N + 63) / 64, off = OFF, end = OFF + 64 * NW;
-  static constexpr int x0 = X0, z0 = Z0;
-  __device__ static int at(int x, int z) { return OFF + (z - Z0) * W_ + (x - X0); }
-};
-using PXX = SP<-4, -2, 67, 16, 0>;          // xx: x in [-4, 62]
-using PXY = SP<-3, -2, 67, 16, PXX::end>;   // xy: x in [-3, 63]
-using PXZ = SP<-3, -4, 67, 19, PXY::end>;   // xz: x in [-3, 63], z in [-4, 14]
-using PYZ = SP<-2, -4, 64, 19, PXZ::end>;   // yz: z in [-4, 14]
-using PZZ = SP<-2, -3, 64, 19, PYZ::end>;   // zz: z in [-3, 15]
-constexpr int NSP = PZZ::end;               // 5888 floats
-constexpr int NWL = NSP / 64;               // wavefront loads per row (92)
-constexpr int NPF = (NWL + LZ - 1) / LZ;    // per wavefront (6)
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const f4 gf4;
 
-struct F2Lds {
-  float s[2][NSP];        // old stresses of row r (two buffers)
-  float v[2][3][NT];      // new u, v, w of row r - 2 at every lane position (two buffers)
-};
-
-// Wavefront load m (wave-uniform) and lane -> field (0..4 = xx, xy, xz,
-// yz, zz), whether the cell is one of the plane's, and its position.
-template <class P>
-__device__ inline bool sp_pos(int e, int &x, int &z) {
-  z = e / P::W;
-  x = e - z * P::W + P::x0;
-  z += P::z0;
-  return e < P::N;
+struct PlaneDef { int z0, rows, base, left, right; };
+__device__ inline PlaneDef plane_def(int f) {
+  switch (f) {
+    case 0: return {-2, PROWS_XY, 0, 4, 3};
+    case 1: return {-2, PROWS_XY, PBASE_XY, 3, 4};
+    case 2: return {-4, PROWS_Z, PBASE_XZ, 3, 4};
+    case 3: return {-4, PROWS_Z, PBASE_YZ, 2, 2};
+    default: return {-3, PROWS_Z, PBASE_ZZ, 2, 2};
+  }
 }
-__device__ inline int sp_cell(int m, int lane, bool &ok, int &x, int &z) {
-  const int h = m * 64 + lane;
-  if (m < PXY::off / 64) { ok = sp_pos<PXX>(h - PXX::off, x, z); return 0; }
-  if (m < PXZ::off / 64) { ok = sp_pos<PXY>(h - PXY::off, x, z); return 1; }
-  if (m < PYZ::off / 64) { ok = sp_pos<PXZ>(h - PXZ::off, x, z); return 2; }
-  if (m < PZZ::off / 64) { ok = sp_pos<PYZ>(h - PYZ::off, x, z); return 3; }
-  ok = sp_pos<PZZ>(h - PZZ::off, x, z);
-  return 4;
+// LDS slot of plane f at (x, z)
+template <int F>
+__device__ inline int at(int x, int z) {
+  constexpr int z0 = F == 0 || F == 1 ? -2 : F == 4 ? -3 : -4;
+  constexpr int base = F == 0 ? 0 : F == 1 ? PBASE_XY : F == 2 ? PBASE_XZ : F == 3 ? PBASE_YZ : PBASE_ZZ;
+  return (base + z - z0) * PW + PX0 + x;
+}
+
+// Wavefront load m (wave-uniform) of lane l: its field (0..4 = xx, xy, xz,
+// yz, zz), the LDS slot and position (x, z) of its chunk's first cell;
+// false if the lane has no chunk.
+__device__ inline bool plane_chunk(int m, int l, int &f, int &slot, int &x, int &z) {
+  f = m < M_XY ? 0 : m < M_XZ ? 1 : m < M_YZ ? 2 : m < M_ZZ ? 3 : 4;
+  const int m0 = f == 0 ? 0 : f == 1 ? M_XY : f == 2 ? M_XZ : f == 3 ? M_YZ : M_ZZ;
+  const PlaneDef d = plane_def(f);
+  const int e = (m - m0) * 64 + l, row = e / PCH, q = e - row * PCH;
+  x = 4 * q - PX0;
+  z = d.z0 + row;
+  slot = (d.base + row) * PW + 4 * q;
+  return row < d.rows;
 }
 
 __global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl, int gx,
                                                        int gz, int gy, int NY) {
-  __shared__ F2Lds L;
+  __shared__ __attribute__((aligned(16))) float Sb[2][NPR * PW];   // old stresses of row r (two buffers)
+  __shared__ float Vb[2][3][NZP * PW];   // new u, v, w of row r - 2 (two buffers)
   // the old set (fo) is set it & 1, the new one the other (see field_pair);
   // only the pointers this kernel uses are selected
   const bool odd = ctl->it & 1;
   const int lid = xcd_remap(blockIdx.x, gx * gz * gy);
   const int bx = lid % gx, bz = (lid / gx) % gz, by = lid / (gx * gz);
-  const int tx = threadIdx.x, ty = threadIdx.y, t = ty * LX + tx;
-  const int wv = __builtin_amdgcn_readfirstlane(ty);   // the wavefront (one lane row)
-  const int i0 = g.nabc + 1 + bx * CX, k0 = g.nabc + 1 + bz * CZ;   // core origin
+  const int tx = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.y);   // the wavefront
+  // the lane's position; lanes 60..63 of the x-ring wavefront have none
+  // (they take position (0, 0) for their reads and write nothing)
+  int x = tx, z = wv - 2;
+  bool has = true;
+  if (wv >= NZP) {
+    const int col = tx & 3;
+    x = col < 2 ? col - 2 : LX + col - 2;
+    z = (tx >> 2) - 2;
+    has = tx < 4 * NZP;
+    if (!has) x = 0, z = 0;
+  }
+  const int i0 = g.nabc + 1 + bx * LX, k0 = g.nabc + 1 + bz * ZC;   // core origin
   const int j0 = g.nabc + 1 + by * NY, je = min(j0 + NY - 1, g.nyt - 2);
-  const int x = tx - 2, z = ty - 2, i = i0 + x, k = k0 + z;
-  const bool in_xz = i >= 1 && i <= g.nxt && k >= 1 && k <= g.nzt;
-  const bool int_xz = i >= g.nabc + 1 && i <= g.nxt - g.nabc && k >= g.nabc + 1 && k <= g.nzt - g.nfs;
-  const bool core = int_xz && x >= 0 && x < CX && z >= 0 && z < CZ;
+  const int i = i0 + x, k = k0 + z;
+  const bool in_xz = has && i >= 1 && i <= g.nxt && k >= 1 && k <= g.nzt;
+  const bool int_xz = has && i >= g.nabc + 1 && i <= g.nxt - g.nabc && k >= g.nabc + 1 && k <= g.nzt - g.nfs;
+  const bool core = int_xz && wv >= 2 && wv < 2 + ZC;
+  const int t = (z + 2) * PW + PX0 + x;   // the lane's slot in a velocity plane
   const int sy = g.sy;
   const float c1 = c.c1, c2 = c.c2, dth = c.dth;
   // depth-layered medium: the factors of the lane's plane k (interior lanes)
@@ -134,23 +163,27 @@ __global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fiel
       else { a = ld32(nu_, q_); bb = ld32(nv_, q_); cc = ld32(nw_, q_); }            \
     }                                                                               \
   } while (0)
-// plane cells of row j into registers: wavefront wv takes wavefront loads
-// m = wv, wv + LZ, ...; each is one field (a uniform base pointer).  The
-// row-independent part of each cell's offset is formed once (-1: no cell).
-  int poff[NPF];
-  const gfloat *psrc[NPF];   // wave-uniform field of each load
+// plane chunks of row j into registers: wavefront wv takes wavefront
+// loads m = wv, wv + NWV; each is one field (a uniform base pointer).  The
+// row-independent part of each chunk's offset is formed once (-1: the
+// chunk is outside the arrays and reads as 0), and so is its LDS slot
+// (-1: no chunk).  A chunk lies in one padded row when its last cell does.
+  int poff[NPF], pslot[NPF];
+  const float *psrc[NPF];   // wave-uniform field of each load
 #pragma unroll
   for (int n = 0; n < NPF; ++n) {
-    const int m = wv + n * LZ;
+    const int m = wv + n * NWV;
     poff[n] = -1;
-    psrc[n] = (const gfloat *)oxx;
+    pslot[n] = -1;
+    psrc[n] = oxx;
     if (m < NWL) {
-      int px, pz;
-      bool ok;
-      const int f = sp_cell(m, tx, ok, px, pz);
-      psrc[n] = (const gfloat *)(f == 0 ? oxx : f == 1 ? oxy : f == 2 ? oxz : f == 3 ? oyz : ozz);
-      const int gi = i0 + px, gk = k0 + pz;
-      if (ok && gi >= 1 && gi <= g.nxt && gk >= 1 && gk <= g.nzt) poff[n] = I3(g, gi, 1, gk);
+      int px, pz, f, sl;
+      if (plane_chunk(m, tx, f, sl, px, pz)) {
+        pslot[n] = sl;
+        const int gi = i0 + px, gk = k0 + pz;
+        if (gk >= 1 && gk <= g.nzt && g.off0 + gi + 3 <= g.px) poff[n] = I3(g, gi, 1, gk);
+      }
+      psrc[n] = f == 0 ? oxx : f == 1 ? oxy : f == 2 ? oxz : f == 3 ? oyz : ozz;
     }
   }
 #define F2_PLANES(pf, j)                                                            \
@@ -158,7 +191,7 @@ __global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fiel
     const bool jok_ = (j) >= 1 && (j) <= g.nyt;                                     \
     const int jo_ = ((j) - 1) * sy;                                                 \
     _Pragma("unroll") for (int n = 0; n < NPF; ++n)                                 \
-      pf[n] = (jok_ && poff[n] >= 0) ? *(const gfloat *)((const __attribute__((address_space(1))) char *)psrc[n] + ((unsigned)(poff[n] + jo_) << 2)) : 0.0f; \
+      pf[n] = (jok_ && poff[n] >= 0) ? *(gf4 *)((const __attribute__((address_space(1))) char *)psrc[n] + ((unsigned)(poff[n] + jo_) << 2)) : f4{0.0f, 0.0f, 0.0f, 0.0f}; \
   } while (0)
 
   // y windows of the old stresses: xy rows r-2..r, yy rows r-2..r+1, yz
@@ -178,7 +211,7 @@ __global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fiel
   // Two prefetch sets, loaded two rows ahead and used alternately: the set
   // of row R holds its plane cells, xy(R+1), yy(R+2), yz(R+1) of the lane's
   // column and the velocity source of row R.
-  float pfA[NPF], pfB[NPF];
+  f4 pfA[NPF], pfB[NPF];
   float qxyA = 0, qyyA = 0, qyzA = 0, quA, qvA, qwA, qxyB = 0, qyyB = 0, qyzB = 0, quB, qvB, qwB;
 #define F2_FETCH(P, R)                                                              \
   do {                                                                              \
@@ -190,14 +223,14 @@ __global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fiel
 #define F2_ITER(P)                                                                  \
   do {                                                                              \
     const int buf = r & 1;                                                          \
-    float *S = L.s[buf];                                                            \
-    float(*V)[NT] = L.v[buf];                                                       \
+    float *S = Sb[buf];                                                             \
+    float(*V)[NZP * PW] = Vb[buf];                                                  \
     /* 1. planes of row r into LDS; new velocity of row r-2 into the velocity plane */ \
     _Pragma("unroll") for (int n = 0; n < NPF; ++n) {                               \
-      const int m = wv + n * LZ;                                                    \
-      if (m < NWL) S[m * 64 + tx] = pf##P[n];                                       \
+      const int m = wv + n * NWV;                                                   \
+      if (m < NWL && pslot[n] >= 0) *(f4 *)&S[pslot[n]] = pf##P[n];                 \
     }                                                                               \
-    V[0][t] = uB; V[1][t] = vC; V[2][t] = wB;                                       \
+    if (has) { V[0][t] = uB; V[1][t] = vC; V[2][t] = wB; }                          \
     const float xyp1 = qxy##P, yyp2 = qyy##P, yzp1 = qyz##P;                        \
     const float uo = qu##P, vo = qv##P, wo = qw##P;                                 \
     __syncthreads();                                                                \
@@ -210,11 +243,11 @@ __global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fiel
 #define F2_COMPUTE()                                                                \
   do {                                                                              \
     float un = uo, vn = vo, wn = wo;                                                \
-    const float xx0 = S[PXX::at(x, z)], xz0 = S[PXZ::at(x, z)], zz0 = S[PZZ::at(x, z)]; \
+    const float xx0 = S[at<0>(x, z)], xz0 = S[at<2>(x, z)], zz0 = S[at<4>(x, z)];   \
     if (int_xz && r >= jlo && r <= jhi) {                                           \
-      const int ax = PXX::at(x, z), ay = PXY::at(x, z), az = PXZ::at(x, z);          \
-      const int ayz = PYZ::at(x, z), azz = PZZ::at(x, z);                           \
-      constexpr int WXZ = PXZ::W, WYZ = PYZ::W, WZZ = PZZ::W;                       \
+      const int ax = at<0>(x, z), ay = at<1>(x, z), az = at<2>(x, z);               \
+      const int ayz = at<3>(x, z), azz = at<4>(x, z);                               \
+      constexpr int WXZ = PW, WYZ = PW, WZZ = PW;                                   \
       un = uo + b * (c1 * (xx0 - S[ax - 1]) + c2 * (S[ax + 1] - S[ax - 2]) +         \
                      c1 * (xy0 - xym1) + c2 * (xyp1 - xym2) +                       \
                      c1 * (xz0 - S[az - WXZ]) + c2 * (S[az + WXZ] - S[az - 2 * WXZ])); \
@@ -235,16 +268,16 @@ __global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fiel
       const float u0 = uB, v0 = vC, w0 = wB;                                        \
       const float dux1 = c1 * (Lu[t + 1] - u0), dux2 = c2 * (Lu[t + 2] - Lu[t - 1]); \
       const float dvy1 = c1 * (v0 - vB), dvy2 = c2 * (vD - vA);                      \
-      const float dwz1 = c1 * (w0 - Lw[t - LX]), dwz2 = c2 * (Lw[t + LX] - Lw[t - 2 * LX]); \
+      const float dwz1 = c1 * (w0 - Lw[t - PW]), dwz2 = c2 * (Lw[t + PW] - Lw[t - 2 * PW]); \
       const int q = qb + (s - 1) * sy;                                              \
-      st32(nxx, q, (xx2 + da * (dux1 + dux2)) + db * (((dvy1 + dvy2) + dwz1) + dwz2)); \
-      st32(nyy, q, (yym2 + da * (dvy1 + dvy2)) + db * (((dux1 + dux2) + dwz1) + dwz2)); \
-      st32(nzz, q, (zz2 + da * (dwz1 + dwz2)) + db * (((dux1 + dux2) + dvy1) + dvy2)); \
-      st32(nxy, q, xym2 + dxy * (((c1 * (uC - u0) + c2 * (un - uA)) + c1 * (v0 - Lv[t - 1])) + \
+      st32_nt(nxx, q, (xx2 + da * (dux1 + dux2)) + db * (((dvy1 + dvy2) + dwz1) + dwz2)); \
+      st32_nt(nyy, q, (yym2 + da * (dvy1 + dvy2)) + db * (((dux1 + dux2) + dwz1) + dwz2)); \
+      st32_nt(nzz, q, (zz2 + da * (dwz1 + dwz2)) + db * (((dux1 + dux2) + dvy1) + dvy2)); \
+      st32_nt(nxy, q, xym2 + dxy * (((c1 * (uC - u0) + c2 * (un - uA)) + c1 * (v0 - Lv[t - 1])) + \
                                    c2 * (Lv[t + 1] - Lv[t - 2])));                  \
-      st32(nxz, q, xz2 + dxz * (((c1 * (Lu[t + LX] - u0) + c2 * (Lu[t + 2 * LX] - Lu[t - LX])) + \
+      st32_nt(nxz, q, xz2 + dxz * (((c1 * (Lu[t + PW] - u0) + c2 * (Lu[t + 2 * PW] - Lu[t - PW])) + \
                                    c1 * (w0 - Lw[t - 1])) + c2 * (Lw[t + 1] - Lw[t - 2]))); \
-      st32(nyz, q, yzm2 + dxz * (((c1 * (Lv[t + LX] - v0) + c2 * (Lv[t + 2 * LX] - Lv[t - LX])) + \
+      st32_nt(nyz, q, yzm2 + dxz * (((c1 * (Lv[t + PW] - v0) + c2 * (Lv[t + 2 * PW] - Lv[t - PW])) + \
                                     c1 * (wC - w0)) + c2 * (wn - wA)));             \
     }                                                                               \
     xx2 = xx1; xx1 = xx0; xz2 = xz1; xz1 = xz0; zz2 = zz1; zz1 = zz0;               \
@@ -281,9 +314,9 @@ hipError_t launch_fused(const Problem &p, hipStream_t s) {
   if (g.nyt - 2 - g.nabc <= 0) return hipSuccess;
   const int ny = p.tune.ny_vel;
   if (ny < 1) return hipErrorInvalidValue;
-  const int gx = cdiv(g.nxt - 2 * g.nabc, CX), gz = cdiv(g.nzt - g.nfs - g.nabc, CZ);
+  const int gx = cdiv(g.nxt - 2 * g.nabc, LX), gz = cdiv(g.nzt - g.nfs - g.nabc, ZC);
   const int gy = cdiv(g.nyt - 2 - g.nabc, ny);
-  hipLaunchKernelGGL(k_vs_fused, dim3(gx * gz * gy), dim3(LX, LZ), 0, s, g, p.c, p.f[0], p.f[1], p.ctl, gx, gz, gy,
+  hipLaunchKernelGGL(k_vs_fused, dim3(gx * gz * gy), dim3(LX, NWV), 0, s, g, p.c, p.f[0], p.f[1], p.ctl, gx, gz, gy,
                      ny);
   return hipGetLastError();
 }

@@ -90,6 +90,7 @@ int main(int argc, char **argv) {
   run<8, 0>("read_only", arr, reps);
   run<9, 3>("velocity_sweep_mix", arr, reps);
   run<9, 6>("stress_sweep_mix", arr, reps);
+  run<9, 9>("fused_sweep_mix", arr, reps);
   run<15, 12>("pml_velocity_mix", arr, reps);
   run<18, 21>("pml_stress_mix", arr, reps);
   for (auto p : arr) (void)hipFree(p);

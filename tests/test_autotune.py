@@ -46,3 +46,26 @@ def test_batch_plan_minimises_summed_time(cache):
     tz, plan = A.batch_plan([a, b])
     assert tz == 8                      # 3.5 + 4.2 + 0.2 + 0.3 < 3 + 4 + 1 + 1
     assert plan[0] == (1, 8, 32, 8, 32) and plan[1] == (1, 8, 32, 8, 32)
+
+
+def test_choose_takes_fused_sweep_when_faster(cache):
+    case = C.c2_large_sw()   # full space
+    tab = {"vel": {"16x32": 3.0, "w8x32": 2.5}, "str": {"16x32": 4.0, "w8x16": 4.1},
+           "pml": {"flat": 3.0}, "fused": {"64": 5.0, "256": 4.8}}
+    json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
+    # 4.8 < 2.5 + 4.0: fused (bit 2) with its best rows per block
+    assert A.choose(case) == (1 | 8 | 4, 8, 256, 16, 32)
+    # the batch engine runs separate sweeps only
+    assert A.choose(case, tz=16) == (1, 16, 32, 16, 32)
+    tab["fused"] = {"64": 7.0}
+    json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
+    assert A.choose(case) == (1 | 8, 8, 32, 16, 32)
+
+
+def test_fused_never_chosen_with_free_surface(cache):
+    case = C.c1_plain_sw()
+    if case.fspace:
+        case = case.replace(fspace=False)
+    tab = {"vel": {"16x32": 3.0}, "str": {"16x32": 4.0}, "pml": {}, "fused": {"64": 1.0}}
+    json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
+    assert A.choose(case)[0] & 4 == 0
