@@ -224,9 +224,11 @@ int dalloc(fd3d_handle *h, T **ptr, size_t n, size_t skew = 0) {
 }
 
 // Start offset of the q-th 3D field: odd multiples of 256 bytes modulo 64 KiB.
+// The unit ($FD3D_FIELD_SKEW, bytes) is rounded down to whole 256-byte
+// units: the sweeps' row segments and 16-byte loads rely on that alignment.
 size_t field_skew(int q) {
   static const char *env = std::getenv("FD3D_FIELD_SKEW");
-  const size_t unit = env ? (size_t)std::atol(env) : 2304;
+  const size_t unit = env ? ((size_t)std::max(0L, std::atol(env)) / 256) * 256 : 2304;
   return (unit * (size_t)q) % 65536;
 }
 
