@@ -20,8 +20,8 @@
 //   1. the old-stress planes of row r (loaded into registers during
 //      iteration r - 2) go into one of two LDS buffers, and every lane puts
 //      its new velocity of row r - 2 (from its register window) into one of
-//      two velocity planes; one barrier;
-//   2. the loads of row r + 2 are issued; every lane forms the new velocity
+//      two velocity planes; the loads of row r + 2 are issued; one barrier;
+//   2. every lane forms the new velocity
 //      of its position at row r from the stress planes (x and z neighbours)
 //      and its register windows of xy, yy, yz (y neighbours); core lanes
 //      form the new stresses of row r - 2 from the velocity plane (x and z
@@ -233,9 +233,9 @@ __global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fiel
     if (has) { V[0][t] = uB; V[1][t] = vC; V[2][t] = wB; }                          \
     const float xyp1 = qxy##P, yyp2 = qyy##P, yzp1 = qyz##P;                        \
     const float uo = qu##P, vo = qv##P, wo = qw##P;                                 \
-    __syncthreads();                                                                \
-    /* 2. loads of row r+2 into this set */                                         \
+    /* 2. loads of row r+2 into this set, in flight across the barrier */           \
     if (r + 2 <= r1) F2_FETCH(P, r + 2);                                            \
+    __syncthreads();                                                                \
     F2_COMPUTE();                                                                   \
   } while (0)
 // New velocity of the lane's position at row r, new stresses of row r-2,
