@@ -1,0 +1,82 @@
+"""Full-size checks of the benchmark problems, through properties that do
+not depend on running the oracle at that size.
+
+The 1024x512x512 slip-weakening rupture (1064x532x552 cells with its
+20-node PML) runs, as benchmarked, as one fused velocity + stress interior
+sweep replayed from a captured step graph.  The separate sweeps run
+kernel by kernel are bit-exact against the oracle on the small cases
+(test_gpu_parity.py), and both forms keep the reference's per-point
+arithmetic, so from the same random wavefield they must agree bit for
+bit after a few steps, in every cell of every field and every output
+slab.  The rate-and-state problem (512x256x256, free surface) is checked
+the same way across launch shapes and between graph replay and kernel by
+kernel launches.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from fd3d_tsn_amd import autotune, capi
+from fd3d_tsn_amd import cases as C
+
+pytestmark = pytest.mark.gpu
+
+OUTS = ("sliprateX", "sliprateZ", "shearstressX", "shearstressZ", "psi")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def random_wavefield(solvers, case, seed, vel_scale, str_scale):
+    """The same random wavefield (every cell non-zero) into each solver,
+    one field at a time."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shape = (case.nxt, case.nyt, case.nzt)
+    for n in capi.FIELDS:
+        scale = vel_scale if n in ("u", "v", "w") else str_scale
+        a = (rng.standard_normal(case.cells, dtype=np.float32) * np.float32(scale)).reshape(shape, order="F")
+        for s in solvers:
+            s.set_field(n, a)
+        del a
+
+
+def assert_same(a, b, case, steps):
+    ra, rb = a.step(steps), b.step(steps)
+    for n in OUTS:
+        if n in ra:
+            assert np.array_equal(ra[n], rb[n]), f"output {n}"
+    assert np.array_equal(ra["seis"], rb["seis"]), "stations"
+    for n in capi.FIELDS:
+        fa, fb = a.field(n), b.field(n)
+        bad = np.count_nonzero(fa != fb)
+        assert bad == 0, f"field {n}: {bad} of {case.cells} cells differ"
+        del fa, fb
+
+
+def test_large_sw_fused_graph_equals_separate_kernel_by_kernel():
+    case = C.c2_large_sw()
+    say(f"[fullsize] {case.name} {case.nxt}x{case.nyt}x{case.nzt}: fused (graph) against separate (kernel by kernel)")
+    with capi.Solver(case) as a, capi.Solver(case) as b:
+        assert autotune.apply(a), "the size class is in the tuning cache"
+        ta = a.get_tuning()
+        assert ta[0] & 4, f"the fused sweep is the benchmarked path (tuning {ta})"
+        b.set_tuning(1, 16, 32, 16, 32)
+        b.set_profiling(True)
+        random_wavefield((a, b), case, seed=41, vel_scale=1e-2, str_scale=1e6)
+        assert_same(a, b, case, steps=3)
+
+
+def test_fvw_launch_shapes_and_graph_agree():
+    case = C.c3_fvw()
+    say(f"[fullsize] {case.name} {case.nxt}x{case.nyt}x{case.nzt}: two launch shapes, graph against kernel by kernel")
+    with capi.Solver(case) as a, capi.Solver(case) as b:
+        a.set_tuning(1 | 8 | 16, 4, 128, 8, 16)
+        b.set_tuning(1, 16, 32, 4, 64)
+        b.set_profiling(True)
+        # small perturbations of the initial state: every cell non-zero, the
+        # fault far from Newton trouble
+        random_wavefield((a, b), case, seed=43, vel_scale=1e-6, str_scale=1e3)
+        assert_same(a, b, case, steps=4)
