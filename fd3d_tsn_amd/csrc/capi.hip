@@ -51,7 +51,14 @@ Dims make_dims(const fd3d_config &c) {
 }
 
 // Default launch shape: tall x-z tiles where the grid allows, and enough
-// y chunks for about 2048 blocks (8 per CU) on large grids.
+// y chunks for about 2048 blocks (8 per CU) on large grids.  Full-space
+// grids with at least kFusedTiles fused-sweep tiles (64 x 11 x-z cells
+// each) run the fused interior sweep, in y chunks of at least 64 rows for
+// about 1500 blocks: measured faster than the separate sweeps on the
+// 1024x512x512 grid (752 tiles; 4.85 against 5.85 ms), slower on the
+// 128x64x64 one (12 tiles).
+constexpr long long kFusedTiles = 512;
+
 Tuning default_tuning(const Dims &g) {
   Tuning t{};
   t.mode = 1;
@@ -64,6 +71,14 @@ Tuning default_tuning(const Dims &g) {
   if (rows < 8) rows = 8;
   t.tz_vel = t.tz_str = tz;
   t.ny_vel = t.ny_str = rows;
+  const long long ftiles = (long long)((nx + 63) / 64) * ((nz + 10) / 11);
+  if (g.fspace && ny > 0 && ftiles >= kFusedTiles) {
+    long long fgy = (1500 + ftiles - 1) / ftiles;
+    int frows = (int)((ny + fgy - 1) / fgy);
+    if (frows < 64) frows = 64;
+    t.mode = 1 | 4;
+    t.ny_vel = frows;
+  }
   return t;
 }
 

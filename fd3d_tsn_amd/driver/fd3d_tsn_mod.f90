@@ -57,6 +57,12 @@ module fd3d_tsn_mod
       type(c_ptr), value :: h
       real(c_float), intent(out) :: out7(*)
     end function
+    integer(c_int) function fd3d_set_tuning(h, mode, tz_vel, ny_vel, tz_str, ny_str) &
+        bind(C, name='fd3d_set_tuning')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: mode, tz_vel, ny_vel, tz_str, ny_str
+    end function
     function fd3d_last_error(h) bind(C, name='fd3d_last_error')
       import :: c_ptr
       type(c_ptr), value :: h
