@@ -52,8 +52,8 @@ constexpr int PX0 = 4;             // its column of x = 0
 
 // Old-stress planes of one row.  Plane f covers the rows z in [z0, z0 +
 // rows) that the velocity stencils of the positions reach, at LDS rows
-// base .. base + rows - 1 (PW floats each), and the columns x in [-left,
-// 63 + right]:
+// base .. base + rows - 1 (PW floats each).  The stencils reach these
+// columns of each plane:
 //   xx: z in [-2, 12], x in [-4, 66]      xy: z in [-2, 12], x in [-3, 67]
 //   xz: z in [-4, 13], x in [-3, 67]      yz: z in [-4, 13], x in [-2, 65]
 //   zz: z in [-3, 14], x in [-2, 65]
@@ -74,14 +74,14 @@ static_assert(4 * NZP <= 64, "the x-ring fits one wavefront");
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const f4 gf4;
 
-struct PlaneDef { int z0, rows, base, left, right; };
+struct PlaneDef { int z0, rows, base; };
 __device__ inline PlaneDef plane_def(int f) {
   switch (f) {
-    case 0: return {-2, PROWS_XY, 0, 4, 3};
-    case 1: return {-2, PROWS_XY, PBASE_XY, 3, 4};
-    case 2: return {-4, PROWS_Z, PBASE_XZ, 3, 4};
-    case 3: return {-4, PROWS_Z, PBASE_YZ, 2, 2};
-    default: return {-3, PROWS_Z, PBASE_ZZ, 2, 2};
+    case 0: return {-2, PROWS_XY, 0};
+    case 1: return {-2, PROWS_XY, PBASE_XY};
+    case 2: return {-4, PROWS_Z, PBASE_XZ};
+    case 3: return {-4, PROWS_Z, PBASE_YZ};
+    default: return {-3, PROWS_Z, PBASE_ZZ};
   }
 }
 // LDS slot of plane f at (x, z)
