@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--workload", default="large_sw", choices=["large_sw", "plain_sw", "fvw", "batched", "served"])
     ap.add_argument("--batch-mode", default="staged", choices=["persistent", "staged"])
     ap.add_argument("--batch-tz", type=int, default=16)
+    ap.add_argument("--batch-fused", default="on", choices=["on", "off"],
+                    help="full-space members of a batch run the fused interior sweep")
     ap.add_argument("--batch-n", type=int, default=64, help="domains of the batched workload")
     ap.add_argument("--serve-n", type=int, default=48, help="requests of the served stream sample")
     ap.add_argument("--serve-slots", type=int, default=32)
@@ -149,7 +151,7 @@ def bench_batched(a):
     from fd3d_tsn_amd import capi
     from fd3d_tsn_amd import cases as C
     cases = C.c4_batched(a.batch_n)
-    tz, plan = autotune.batch_plan(cases)
+    tz, plan = autotune.batch_plan(cases, fused=a.batch_fused == "on")
     tuned = sum(p is not None for p in plan)
     nsteps_run = cases[0].ntfd
     t0 = time.perf_counter()
@@ -173,7 +175,10 @@ def bench_batched(a):
         s.step(a.steps, outputs=False)
         s.sync()
         plain.append((time.perf_counter() - t1) / a.steps)
-    step_b = [MOVED_BULK * nb + MOVED_PML * npml for _, nb, npml in (s.step_bytes() for s in solvers)]
+    # bytes each member moves per step in the batch (fused members: 72 B per bulk cell)
+    step_b = [((MOVED_BULK_FUSED if p is not None and p[0] & 4 and c.fspace else MOVED_BULK) * nb + MOVED_PML * npml)
+              for c, p, (_, nb, npml) in zip(cases, plan, (s.step_bytes() for s in solvers))]
+    step_plain = [MOVED_BULK * nb + MOVED_PML * npml for _, nb, npml in (s.step_bytes() for s in solvers)]
     for s, p in zip(solvers, plan):
         s.init()
         if p is not None:
@@ -206,13 +211,14 @@ def bench_batched(a):
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"batched_{len(cases)}_domains_96-320", "domains": len(cases),
                       "cells": cells, "steps_per_rupture": nsteps_run, "mode": a.batch_mode,
-                      "tz": tz, "autotuned_members": tuned},
+                      "tz": tz, "autotuned_members": tuned,
+                      "fused_members": sum(1 for c, p in zip(cases, plan) if p is not None and p[0] & 4 and c.fspace)},
            "batch": {"gpu_ms_per_step": gpu_ms / max(n, 1), "Gcell_updates_per_s": cells / wall / 1e9,
                      "hbm_frac": step_bytes / wall / 1e9 / HBM_PEAK_GBS, "info": info},
            "plain_back_to_back": {"ms_per_step_sum": t_plain * 1e3,
                                   "ruptures_per_s": len(cases) / (nsteps_run * t_plain),
                                   "Gcell_updates_per_s": cells / t_plain / 1e9,
-                                  "hbm_frac": step_bytes / t_plain / 1e9 / HBM_PEAK_GBS,
+                                  "hbm_frac": sum(step_plain) / t_plain / 1e9 / HBM_PEAK_GBS,
                                   "setup_s_per_rupture": sum(setup_s) / len(setup_s),
                                   "setup_phases_s": phases},
            "speedup_vs_plain": t_plain / wall}
@@ -237,7 +243,7 @@ def bench_served(a):
     t1 = time.perf_counter()
     plain = serve.run_plain(reqs)
     print(f"[bench] plain back-to-back {time.perf_counter()-t1:.1f} s", file=sys.stderr, flush=True)
-    srv = serve.Server(max_slots=a.serve_slots, chunk=a.serve_chunk, tz=a.batch_tz)
+    srv = serve.Server(max_slots=a.serve_slots, chunk=a.serve_chunk, tz=a.batch_tz, fused=a.batch_fused == "on")
     t1 = time.perf_counter()
     got = srv.run(reqs)
     print(f"[bench] served {time.perf_counter()-t1:.1f} s", file=sys.stderr, flush=True)

@@ -180,10 +180,31 @@ def apply(solver: "capi.Solver", tz: Optional[int] = None) -> bool:
     return True
 
 
-def batch_plan(cases: Iterable[C.Case]) -> Tuple[int, List[Optional[Tuple[int, int, int, int, int]]]]:
+FUSED_BATCH_NY = 128
+
+
+def batch_member(case: C.Case, t: Optional[Tuple[int, int, int, int, int]], tz: int,
+                 fused: bool = True) -> Optional[Tuple[int, int, int, int, int]]:
+    """A member's shape in a batch: t (from choose at the batch's tz) and,
+    for full-space members when fused is set, the fused interior sweep
+    (mode bit 2) in y chunks of at most FUSED_BATCH_NY rows.  In a batch
+    the fused tiles of all members fill the GPU together, so it pays even
+    for grids too small to fill it alone; chunks of a bounded length keep
+    members of different lengths balanced."""
+    if not (fused and case.fspace):
+        return t
+    nyi = case.nyt - 2 - case.nabc
+    if nyi <= 0:
+        return t
+    base = t if t is not None else (1, tz, 32, tz, 32)
+    return (base[0] | 4, base[1], min(nyi, FUSED_BATCH_NY), base[3], base[4])
+
+
+def batch_plan(cases: Iterable[C.Case], fused: bool = True) -> Tuple[int, List[Optional[Tuple[int, int, int, int, int]]]]:
     """Common tile height for a batch and each member's shape at that height:
     the height with the least summed sweep time over the measured members,
-    among the heights measured for every one of them (16 when none is)."""
+    among the heights measured for every one of them (16 when none is);
+    full-space members run the fused sweep when fused is set (batch_member)."""
     cases = list(cases)
     tables = [e for e in (lookup(c) for c in cases) if e]
     totals = {}
@@ -198,7 +219,7 @@ def batch_plan(cases: Iterable[C.Case]) -> Tuple[int, List[Optional[Tuple[int, i
         if tot is not None and tables:
             totals[tz] = tot
     tz = min(totals, key=totals.get) if totals else 16
-    return tz, [choose(c, tz) for c in cases]
+    return tz, [batch_member(c, choose(c, tz), tz, fused) for c in cases]
 
 
 def main():

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "handle.h"
+#include "fused.h"
 #include "stages.h"
 #include "tiles.h"
 
@@ -53,7 +54,7 @@ struct BatchDomain {
   int fix;            // 64-wide blocks along x over the fault interior
   int nux;            // 64-wide blocks along x over the nucleation patch
   int nout;           // fault-node items of the output stage; station items follow
-  int pad;
+  int fgx, fgz, nyf;  // fused sweep: tiles along x and z, rows per item
 };
 
 struct BatchSeg {
@@ -213,6 +214,18 @@ __global__ void __launch_bounds__(64 * TZ) k_batch_persistent(BatchArgs A) {
   }
 }
 
+// The fused interior sweep of every member that runs it: one 64 x 16
+// workgroup per fused tile and y chunk (fused.h).
+__global__ void __launch_bounds__(fused::NT) k_batch_fused(BatchArgs A, int seg_lo, int seg_hi, int item0, int n) {
+  __shared__ fused::FusedLds L;
+  const int item = item0 + xcd_remap((int)blockIdx.x, n);
+  const int sg = find_seg(A.seg, seg_lo, seg_hi, item);
+  const BatchDomain &D = A.dom[A.seg[sg].dom];
+  const int b = item - A.seg[sg].start;
+  fused::fused_body(D.p.g, D.p.c, D.p.f[0], D.p.f[1], D.p.ctl, b % D.fgx, (b / D.fgx) % D.fgz, b / (D.fgx * D.fgz),
+                    D.nyf, L);
+}
+
 // One stage of every member; the stage is a template argument, so each
 // instantiation holds (and allocates registers for) one stage body only.
 template <int TZ, bool FVW, int STAGE>
@@ -303,8 +316,11 @@ int bfail(fd3d_batch *b, int code, const std::string &m) {
                    std::string(#call) + ": " + hipGetErrorString(e_));                       \
   } while (0)
 
-// Work items of each stage of h at tile height tz.
-BatchDomain describe(const fd3d_handle *h, int tz) {
+// Work items of each stage of h at tile height tz.  With staged launches a
+// full-space member whose tuning asks for the fused interior sweep (mode
+// bit 2) runs it as its own stage; otherwise the member runs the separate
+// sweeps (work items of one sweep), whatever form it uses alone.
+BatchDomain describe(const fd3d_handle *h, int tz, bool staged) {
   BatchDomain D{};
   D.p = h->p;
   const Dims &g = h->p.g;
@@ -321,6 +337,10 @@ BatchDomain describe(const fd3d_handle *h, int tz) {
   const int nxn = g.nxt - g.nabc - (g.nabc - 1) + 1, nzn = g.nzt - g.nfs - (g.nabc - 1) + 1;
   D.nux = cdiv(nxn, 64);
   D.nout = D.fix * cdiv(nk, tz);
+  const bool fz = staged && (h->p.tune.mode & 4) && g.fspace && nyi > 0;
+  D.fgx = cdiv(ni, fused::LX);
+  D.fgz = cdiv(nk, fused::ZC);
+  D.nyf = std::max(1, h->p.tune.ny_vel);
   int items[K_COUNT] = {0};
   items[K_VEL_INTERIOR] = D.gx * D.gz * D.gyv;
   items[K_VEL_FAULT] = D.fpx * cdiv(g.nzt, tz);
@@ -335,15 +355,14 @@ BatchDomain describe(const fd3d_handle *h, int tz) {
   items[K_NUCLEATION] = D.nux * cdiv(nzn, tz);
   items[K_ADVANCE] = 1;
   items[K_RATES] = h->p.rates.on ? 1 + cdiv(h->p.rates.NL * h->p.rates.NW, blk) : 0;
+  items[K_FUSED] = fz ? D.fgx * D.fgz * cdiv(nyi, D.nyf) : 0;
   for (int s = 0; s < K_COUNT; ++s) { D.n[s] = 0; D.dep[s] = -1; }
-  // the batch runs the separate interior sweeps (its work items are tiles
-  // of one sweep), whatever form the problem uses alone
   std::vector<KernelId> seq;
   for (KernelId id : h->seq) {
-    if (id == K_FUSED) { seq.push_back(K_VEL_INTERIOR); continue; }
+    if (id == K_FUSED && !fz) { seq.push_back(K_VEL_INTERIOR); continue; }
     seq.push_back(id);
   }
-  {
+  if (!fz) {
     auto it = std::find(seq.begin(), seq.end(), K_STR_FAULT);
     if (std::find(seq.begin(), seq.end(), K_STR_INTERIOR) == seq.end() && it != seq.end())
       seq.insert(it, K_STR_INTERIOR);
@@ -387,7 +406,11 @@ int capture_t(fd3d_batch *b) {
     for (int s = 0; s < K_COUNT; ++s) {
       const int n = b->stage_item0[s + 1] - b->stage_item0[s];
       if (n <= 0) continue;
-      launch_stage_rt<TZ, FVW>(s, n, b->stream, A, b->stage_seg0[s], b->stage_seg0[s + 1] - 1, b->stage_item0[s]);
+      if (s == K_FUSED)
+        hipLaunchKernelGGL(k_batch_fused, dim3(n), dim3(fused::LX, fused::NWV), 0, b->stream, A, b->stage_seg0[s],
+                           b->stage_seg0[s + 1] - 1, b->stage_item0[s], n);
+      else
+        launch_stage_rt<TZ, FVW>(s, n, b->stream, A, b->stage_seg0[s], b->stage_seg0[s + 1] - 1, b->stage_item0[s]);
       ++b->launches;
     }
   }
@@ -400,7 +423,7 @@ int rebuild(fd3d_batch *b) {
   std::vector<BatchDomain> doms;
   b->fvw = false;
   for (fd3d_handle *h : b->mem) {
-    doms.push_back(describe(h, b->tz));
+    doms.push_back(describe(h, b->tz, b->mode == FD3D_BATCH_STAGED));
     if (h->p.fv.fvw) b->fvw = true;
   }
   b->segs.clear();

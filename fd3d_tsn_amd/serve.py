@@ -124,8 +124,9 @@ class Server:
 
     def __init__(self, max_slots: int = 32, chunk: int = 16, tz: int = 16, mode: int = capi.STAGED,
                  reserve_bytes: int = 4 << 30, keep_final: bool = True,
-                 log: Optional[Callable[[str], None]] = None):
+                 log: Optional[Callable[[str], None]] = None, fused: bool = True):
         self.max_slots = max_slots
+        self.fused = fused and mode == capi.STAGED   # full-space members run the fused sweep
         self.chunk = chunk
         self.reserve = reserve_bytes
         self.keep_final = keep_final
@@ -165,7 +166,11 @@ class Server:
                 s = capi.Solver(r.case)
                 if r.bins is not None:
                     s.set_source_bins(*r.bins, nbins=r.steps)
-                autotune.apply(s, self.tz)       # rows per tile for this size class
+                # rows per tile for this size class; the fused sweep for
+                # full-space requests
+                t = autotune.batch_member(r.case, autotune.choose(r.case, self.tz), self.tz, self.fused)
+                if t is not None:
+                    s.set_tuning(*t)
                 ready.append({"req": r, "solver": s, "left": r.steps, "admitted": clock() - t0})
             if queue and not active and not ready:
                 raise RuntimeError(f"request {queue[0].rid} does not fit in device memory")

@@ -43,9 +43,13 @@ def test_batch_plan_minimises_summed_time(cache):
         A.size_class(a): {"vel": {"16x32": 3.0, "8x32": 3.5}, "str": {"16x32": 4.0, "8x32": 4.2}, "pml": {}},
         A.size_class(b): {"vel": {"16x32": 1.0, "8x32": 0.2}, "str": {"16x32": 1.0, "8x32": 0.3}, "pml": {}},
     }}, open(cache, "w"))
-    tz, plan = A.batch_plan([a, b])
+    tz, plan = A.batch_plan([a, b], fused=False)
     assert tz == 8                      # 3.5 + 4.2 + 0.2 + 0.3 < 3 + 4 + 1 + 1
     assert plan[0] == (1, 8, 32, 8, 32) and plan[1] == (1, 8, 32, 8, 32)
+    # with the fused sweep: the full-space member (a) runs it in chunks of at
+    # most 128 rows, the free-surface one (b) keeps the separate sweeps
+    tz, plan = A.batch_plan([a, b])
+    assert plan[0] == (1 | 4, 8, 128, 8, 32) and plan[1] == (1, 8, 32, 8, 32)
 
 
 def test_choose_takes_fused_sweep_when_faster(cache):
@@ -69,3 +73,10 @@ def test_fused_never_chosen_with_free_surface(cache):
     tab = {"vel": {"16x32": 3.0}, "str": {"16x32": 4.0}, "pml": {}, "fused": {"64": 1.0}}
     json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
     assert A.choose(case)[0] & 4 == 0
+
+
+def test_batch_member_fused_chunks():
+    small = C.c1_plain_sw()            # full space, 62 interior rows
+    assert A.batch_member(small, None, 16) == (1 | 4, 16, 62, 16, 32)
+    assert A.batch_member(small, (1, 8, 16, 8, 32), 8, fused=False) == (1, 8, 16, 8, 32)
+    assert A.batch_member(C.c3_fvw(), (1, 8, 16, 8, 32), 8) == (1, 8, 16, 8, 32)   # free surface

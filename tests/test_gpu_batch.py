@@ -113,6 +113,42 @@ def test_batch_equals_alone(mode, tz):
         s.close()
 
 
+@pytest.mark.parametrize("tz,ny", [(16, 128), (4, 3)], ids=["tz16_ny128", "tz4_ny3"])
+def test_batch_with_fused_members_equals_alone(tz, ny):
+    """Full-space members run the fused interior sweep as a stage of the
+    batch (rows per item ny), free-surface members the separate sweeps;
+    every member equals the problem stepped alone, bit for bit."""
+    nsteps, chunk = 48, 16
+    cases = mix()
+    say(f"[batch] fused members, tz {tz}, {ny} rows per item: {len(cases)} problems, {nsteps} steps")
+    refs = [alone(c, nsteps) for c in cases]
+    solvers = [capi.Solver(c) for c in cases]
+    for s, c in zip(solvers, cases):
+        if c.fspace:
+            s.set_tuning(1 | 4, tz, ny, tz, 32)
+    got = [{k: [] for k in KEYS} for _ in cases]
+    with capi.Batch(8, tz=tz, mode=capi.STAGED) as b:
+        for s in solvers:
+            b.add(s)
+        for _ in range(nsteps // chunk):
+            b.step(chunk)
+            b.sync()
+            for q, s in enumerate(solvers):
+                o = s.read_outputs(chunk)
+                for k in KEYS:
+                    got[q][k].append(o[k])
+        info = b.info()
+        say(f"[batch] info {info}")
+        n_stages = len({k for c in cases for k in stage_list(c)}) + 1   # + the fused stage
+        assert info["launches"] == n_stages
+        for q, (s, c) in enumerate(zip(solvers, cases)):
+            res = {k: np.concatenate(v) for k, v in got[q].items()}
+            check_equal(c.name, res, refs[q][0])
+            check_final(c.name, s.final(), refs[q][1])
+    for s in solvers:
+        s.close()
+
+
 def test_members_join_and_leave():
     """Continuous batching: problems join and leave between chunks; each
     result equals the problem run alone for its own number of steps."""
