@@ -187,10 +187,12 @@ def batch_member(case: C.Case, t: Optional[Tuple[int, int, int, int, int]], tz: 
                  fused: bool = True) -> Optional[Tuple[int, int, int, int, int]]:
     """A member's shape in a batch: t (from choose at the batch's tz) and,
     for full-space members when fused is set, the fused interior sweep
-    (mode bit 2) in y chunks of at most FUSED_BATCH_NY rows.  In a batch
-    the fused tiles of all members fill the GPU together, so it pays even
-    for grids too small to fill it alone; chunks of a bounded length keep
-    members of different lengths balanced."""
+    (mode bit 2) in y chunks of at most FUSED_BATCH_NY rows (in ny_vel).
+    In a batch the fused tiles of all members fill the GPU together, so it
+    pays even for grids too small to fill it alone; chunks of a bounded
+    length keep members of different lengths balanced.  When the batch
+    holds too little fused work (fd3d_batch_set_fused_items) it runs the
+    separate sweeps, both with the member's ny_str rows."""
     if not (fused and case.fspace):
         return t
     nyi = case.nyt - 2 - case.nabc

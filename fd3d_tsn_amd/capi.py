@@ -106,6 +106,7 @@ def load(path: Optional[str] = None):
     L.fd3d_batch_step.argtypes = [B, ctypes.c_int]
     L.fd3d_batch_sync.argtypes = [B]
     L.fd3d_batch_info.argtypes = [B, ctypes.POINTER(ctypes.c_int)]
+    L.fd3d_batch_set_fused_items.argtypes = [B, ctypes.c_int]
     L.fd3d_batch_elapsed.restype = ctypes.c_double
     L.fd3d_batch_elapsed.argtypes = [B, ctypes.POINTER(ctypes.c_int)]
     L.fd3d_batch_last_error.restype = ctypes.c_char_p
@@ -385,6 +386,11 @@ class Batch:
 
     def sync(self):
         self._chk(self.L.fd3d_batch_sync(self.b))
+
+    def set_fused_items(self, min_items_per_cu: int):
+        """Least fused work items per CU at which members asking for the
+        fused interior sweep run it in the batch (0: always)."""
+        self._chk(self.L.fd3d_batch_set_fused_items(self.b, min_items_per_cu))
 
     def info(self) -> Dict[str, int]:
         buf = (ctypes.c_int * 4)()

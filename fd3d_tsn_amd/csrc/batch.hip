@@ -292,6 +292,7 @@ struct fd3d_batch {
   int total = 0, workers = 0, launches = 0;
   bool dirty = true;
   bool fvw = false;   // some member uses rate-and-state friction
+  int fused_items = 8;   // least fused work items per CU for the fused stage
   long long wait_ticks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -316,11 +317,12 @@ int bfail(fd3d_batch *b, int code, const std::string &m) {
                    std::string(#call) + ": " + hipGetErrorString(e_));                       \
   } while (0)
 
-// Work items of each stage of h at tile height tz.  With staged launches a
-// full-space member whose tuning asks for the fused interior sweep (mode
-// bit 2) runs it as its own stage; otherwise the member runs the separate
-// sweeps (work items of one sweep), whatever form it uses alone.
-BatchDomain describe(const fd3d_handle *h, int tz, bool staged) {
+// Work items of each stage of h at tile height tz.  With fused allowed
+// (staged launches, enough fused work in the batch) a full-space member
+// whose tuning asks for the fused interior sweep (mode bit 2) runs it as
+// its own stage; otherwise the member runs the separate sweeps (work items
+// of one sweep), whatever form it uses alone.
+BatchDomain describe(const fd3d_handle *h, int tz, bool allow_fused) {
   BatchDomain D{};
   D.p = h->p;
   const Dims &g = h->p.g;
@@ -330,6 +332,9 @@ BatchDomain describe(const fd3d_handle *h, int tz, bool staged) {
   D.gz = cdiv(nk, tz);
   D.nyv = std::max(1, h->p.tune.ny_vel);
   D.nys = std::max(1, h->p.tune.ny_str);
+  // a member that asks for the fused sweep holds its fused rows in ny_vel;
+  // when the batch runs the separate sweeps instead, both use ny_str
+  if ((h->p.tune.mode & 4) && g.fspace) D.nyv = D.nys;
   D.gyv = nyi > 0 ? cdiv(nyi, D.nyv) : 0;
   const int gys = nyi > 0 ? cdiv(nyi, D.nys) : 0;
   D.fpx = cdiv(g.nxt, 64);
@@ -337,7 +342,7 @@ BatchDomain describe(const fd3d_handle *h, int tz, bool staged) {
   const int nxn = g.nxt - g.nabc - (g.nabc - 1) + 1, nzn = g.nzt - g.nfs - (g.nabc - 1) + 1;
   D.nux = cdiv(nxn, 64);
   D.nout = D.fix * cdiv(nk, tz);
-  const bool fz = staged && (h->p.tune.mode & 4) && g.fspace && nyi > 0;
+  const bool fz = allow_fused && (h->p.tune.mode & 4) && g.fspace && nyi > 0;
   D.fgx = cdiv(ni, fused::LX);
   D.fgz = cdiv(nk, fused::ZC);
   D.nyf = std::max(1, h->p.tune.ny_vel);
@@ -422,8 +427,19 @@ int rebuild(fd3d_batch *b) {
   if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
   std::vector<BatchDomain> doms;
   b->fvw = false;
+  bool fused = b->mode == FD3D_BATCH_STAGED;
+  if (fused) {
+    // members that ask for the fused sweep get it when the batch's fused
+    // items fill the GPU, otherwise every member runs the separate sweeps
+    long long items = 0;
+    for (fd3d_handle *h : b->mem) items += describe(h, b->tz, true).n[K_FUSED];
+    int dev = 0, ncu = 0;
+    BCHK(b, hipGetDevice(&dev));
+    BCHK(b, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    fused = items >= (long long)b->fused_items * std::max(ncu, 1);
+  }
   for (fd3d_handle *h : b->mem) {
-    doms.push_back(describe(h, b->tz, b->mode == FD3D_BATCH_STAGED));
+    doms.push_back(describe(h, b->tz, fused));
     if (h->p.fv.fvw) b->fvw = true;
   }
   b->segs.clear();
@@ -574,6 +590,19 @@ int fd3d_batch_sync(fd3d_batch *b) {
     int rc = fd3d_handle_check_ctl(h, b->stream);
     if (rc) return bfail(b, rc, h->err);
   }
+  return 0;
+}
+
+// The fused stage runs one 1024-lane workgroup per CU at a time; with few
+// items per CU its last wave leaves CUs idle for longer than the saved
+// bytes gain (measured: a 64-domain batch of sides 96-320, about 30 items
+// per CU, steps 7% faster fused; the first 48 served requests, 2-4 items
+// per CU, 5% slower).  The default threshold is 8 items per CU.
+int fd3d_batch_set_fused_items(fd3d_batch *b, int min_items_per_cu) {
+  if (!b || min_items_per_cu < 0) return FD3D_E_INVALID;
+  BCHK(b, hipStreamSynchronize(b->stream));
+  b->fused_items = min_items_per_cu;
+  b->dirty = true;
   return 0;
 }
 

@@ -241,6 +241,15 @@ int fd3d_batch_step(fd3d_batch *b, int nsteps);
  * the members' FD3D_E_NEWTON / FD3D_E_NONFINITE flags. */
 int fd3d_batch_sync(fd3d_batch *b);
 
+/* Members whose tuning asks for the fused interior sweep (fd3d_set_tuning
+ * mode bit 2, full-space problems; ny_vel rows per work item) run it as a
+ * stage of the batch when the batch's fused work items reach
+ * min_items_per_cu per compute unit (default 8; 0: whenever a member asks);
+ * otherwise every member runs the separate sweeps, such members with ny_str
+ * rows for both.  Results are bit-identical either way.  Marks the step
+ * graph for re-capture. */
+int fd3d_batch_set_fused_items(fd3d_batch *b, int min_items_per_cu);
+
 /* out4: resident workgroups of the persistent launch, work items per step,
  * (problem, stage) segments per step, kernel launches per step. */
 int fd3d_batch_info(const fd3d_batch *b, int *out4);

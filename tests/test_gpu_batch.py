@@ -128,6 +128,7 @@ def test_batch_with_fused_members_equals_alone(tz, ny):
             s.set_tuning(1 | 4, tz, ny, tz, 32)
     got = [{k: [] for k in KEYS} for _ in cases]
     with capi.Batch(8, tz=tz, mode=capi.STAGED) as b:
+        b.set_fused_items(0)     # these members are too small to reach the default threshold
         for s in solvers:
             b.add(s)
         for _ in range(nsteps // chunk):
@@ -145,6 +146,30 @@ def test_batch_with_fused_members_equals_alone(tz, ny):
             res = {k: np.concatenate(v) for k, v in got[q].items()}
             check_equal(c.name, res, refs[q][0])
             check_final(c.name, s.final(), refs[q][1])
+    for s in solvers:
+        s.close()
+
+
+def test_small_fused_batch_runs_separate_sweeps():
+    """Below the default threshold of fused work the batch runs every member
+    with the separate sweeps, even when their tuning asks for the fused one."""
+    cases = [c for c in mix() if c.fspace]
+    solvers = [capi.Solver(c) for c in cases]
+    for s in solvers:
+        s.set_tuning(1 | 4, 16, 64, 16, 32)
+    with capi.Batch(4, tz=16, mode=capi.STAGED) as b:
+        for s in solvers:
+            b.add(s)
+        b.step(2)
+        b.sync()
+        n_stages = len({k for c in cases for k in stage_list(c)})
+        assert b.info()["launches"] == n_stages
+        b.set_fused_items(0)
+        b.step(2)
+        b.sync()
+        assert b.info()["launches"] == n_stages - 2 + 1   # fused replaces the two sweeps
+        with pytest.raises(capi.FD3DError):
+            b.set_fused_items(-1)
     for s in solvers:
         s.close()
 
