@@ -52,12 +52,14 @@ Dims make_dims(const fd3d_config &c) {
 
 // Default launch shape: tall x-z tiles where the grid allows, and enough
 // y chunks for about 2048 blocks (8 per CU) on large grids.  Full-space
-// grids with at least kFusedTiles fused-sweep tiles (64 x 11 x-z cells
-// each) run the fused interior sweep, in y chunks of at least 64 rows for
-// about 1500 blocks: measured faster than the separate sweeps on the
-// 1024x512x512 grid (752 tiles; 4.85 against 5.85 ms), slower on the
-// 128x64x64 one (12 tiles).
-constexpr long long kFusedTiles = 512;
+// grids with at least kFusedTileRows tile-rows of the fused sweep (its x-z
+// tiles of 64 x 11 cells times the interior rows) run the fused interior
+// sweep, in y chunks of at least 32 rows for about 1024 blocks.  Measured
+// (interior sweeps, fused against separate): 1024x512x512 4.85 against
+// 5.85 ms, 256x256x256 0.36 against 0.45, 128x256x256 0.21 against 0.25;
+// even at 256x128x128 (6048 tile-rows); slower below, 192x128x128 0.107
+// against 0.086, 128x64x64 0.067 against 0.032.
+constexpr long long kFusedTileRows = 8192;
 
 Tuning default_tuning(const Dims &g) {
   Tuning t{};
@@ -72,10 +74,10 @@ Tuning default_tuning(const Dims &g) {
   t.tz_vel = t.tz_str = tz;
   t.ny_vel = t.ny_str = rows;
   const long long ftiles = (long long)((nx + 63) / 64) * ((nz + 10) / 11);
-  if (g.fspace && ny > 0 && ftiles >= kFusedTiles) {
-    long long fgy = (1500 + ftiles - 1) / ftiles;
+  if (g.fspace && ny > 0 && ftiles * ny >= kFusedTileRows) {
+    long long fgy = (1024 + ftiles - 1) / ftiles;
     int frows = (int)((ny + fgy - 1) / fgy);
-    if (frows < 64) frows = 64;
+    if (frows < 32) frows = 32;
     t.mode = 1 | 4;
     t.ny_vel = frows;
   }

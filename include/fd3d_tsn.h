@@ -180,9 +180,9 @@ int fd3d_download_source_bins(fd3d_handle *h, float *msrx, float *msrz, float *m
  * bit 2: one fused velocity + stress pass over the interior (full space
  * only, ny_vel rows per block); mode bits 3 and 4 (with bit 0): velocity
  * and stress tiles 128 wide instead of 64 (tz in {4, 8}).  fd3d_create picks a default from
- * the grid size (the fused sweep for full-space grids of at least 512 of its
- * 64 x 11 x-z tiles, such as 1024x512x512); fd3d_tsn_amd/autotune.py
- * measures and caches the best.
+ * the grid size (the fused sweep for full-space grids whose 64 x 11 x-z
+ * tiles times interior rows reach 8192, such as 256x256x256 and up);
+ * fd3d_tsn_amd/autotune.py measures and caches the best.
  * Results do not depend on the shape (bit-identical). */
 int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str);
 int fd3d_get_tuning(const fd3d_handle *h, int *out5);
