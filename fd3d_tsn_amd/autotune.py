@@ -17,7 +17,8 @@ of sizes and keeps the PML width and the free-surface setting.
 
 For full-space problems the fused velocity + stress sweep (mode bit 2)
 is timed too, at several rows per block; :func:`choose` takes it when it
-beats the best pair of separate sweeps.
+beats the best pair of separate sweeps by 5%, with rows per block from
+the problem's own geometry (:func:`fused_ny`).
 
 * :func:`tune` measures one problem and stores its table;
 * :func:`choose` returns the best (tz, ny) of a problem's class, or the
@@ -150,6 +151,22 @@ def lookup(case: C.Case) -> Optional[Dict]:
     return _load().get("entries", {}).get(size_class(case))
 
 
+FUSED_MARGIN = 0.95
+
+
+def fused_ny(case: C.Case) -> int:
+    """Rows per block of the fused sweep from the problem's own geometry:
+    y chunks for about 1024 workgroups (4 per CU), at least 32 rows.  The
+    fused sweep runs one workgroup per CU at a time, so its time follows
+    the block count of the exact grid; a row count measured on another
+    member of the size class does not carry over (a 248x274x310 grid at the
+    class's 128 rows made 348 blocks and ran 16% slower than separate)."""
+    tiles = -(-case.nxtT // 64) * -(-case.nztT // 11)
+    nyi = case.nyt - 2 - case.nabc
+    fgy = -(-1024 // max(tiles, 1))
+    return max(32, -(-nyi // fgy))
+
+
 def choose(case: C.Case, tz: Optional[int] = None) -> Optional[Tuple[int, int, int, int, int]]:
     """(mode, tz_vel, ny_vel, tz_str, ny_str) from the cache, or None."""
     e = lookup(case)
@@ -164,11 +181,13 @@ def choose(case: C.Case, tz: Optional[int] = None) -> Optional[Tuple[int, int, i
         mode = 3
     # the fused sweep (single problems only: the batch engine runs the
     # separate sweeps) when it beats the best separate pair
+    # (rows per block from the problem's geometry, fused_ny) when its best
+    # measured time beats the best separate pair by FUSED_MARGIN
     fz = e.get("fused")
     if tz is None and fz and case.fspace:
-        fny, fms = min(((int(k), v) for k, v in fz.items()), key=lambda kv: kv[1])
-        if fms < _best(e["vel"])[3] + _best(e["str"])[3]:
-            return (mode | 4, tv, fny, ts, ns)
+        fms = min(fz.values())
+        if fms < FUSED_MARGIN * (_best(e["vel"])[3] + _best(e["str"])[3]):
+            return (mode | 4, tv, fused_ny(case), ts, ns)
     return (mode, tv, nv, ts, ns)
 
 

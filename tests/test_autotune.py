@@ -57,13 +57,22 @@ def test_choose_takes_fused_sweep_when_faster(cache):
     tab = {"vel": {"16x32": 3.0, "w8x32": 2.5}, "str": {"16x32": 4.0, "w8x16": 4.1},
            "pml": {"flat": 3.0}, "fused": {"64": 5.0, "256": 4.8}}
     json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
-    # 4.8 < 2.5 + 4.0: fused (bit 2) with its best rows per block
-    assert A.choose(case) == (1 | 8 | 4, 8, 256, 16, 32)
+    # 4.8 < 0.95 (2.5 + 4.0): fused (bit 2), rows per block from the grid
+    # (752 fused tiles, 510 rows: 2 chunks of 255)
+    assert A.fused_ny(case) == 255
+    assert A.choose(case) == (1 | 8 | 4, 8, 255, 16, 32)
     # the batch engine runs separate sweeps only
     assert A.choose(case, tz=16) == (1, 16, 32, 16, 32)
-    tab["fused"] = {"64": 7.0}
+    tab["fused"] = {"64": 6.3}          # within the 5% margin of 6.5: separate
     json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
     assert A.choose(case) == (1 | 8, 8, 32, 16, 32)
+
+
+def test_fused_ny_targets_about_1024_blocks():
+    c = C.c2_large_sw().replace(nxtT=248, nytT=274, nztT=310)   # 116 tiles, 272 rows
+    assert A.fused_ny(c) == 32                                  # 9 chunks: 1044 blocks
+    c = C.c2_large_sw().replace(nxtT=256, nytT=256, nztT=256)   # 96 tiles, 254 rows
+    assert A.fused_ny(c) == 32
 
 
 def test_fused_never_chosen_with_free_surface(cache):
