@@ -15,8 +15,7 @@ Measured tables are cached per device and size class in a JSON file
 size class rounds each interior extent to the nearest of a fixed ladder
 of sizes and keeps the PML width and the free-surface setting.
 
-For full-space problems the fused velocity + stress sweep (mode bit 2)
-is timed too, at several rows per block; :func:`choose` takes it when it
+The fused velocity + stress sweep (mode bit 2) is timed too, at several rows per block; :func:`choose` takes it when it
 beats the best pair of separate sweeps by 5%, with rows per block from
 the problem's own geometry (:func:`fused_ny`).
 
@@ -95,23 +94,26 @@ def tune(solver: "capi.Solver", steps: int = 3, warmup: int = 1, save: bool = Tr
     """Time every candidate shape on a live problem (its state advances by
     the steps taken; results do not depend on the shape).  Returns and
     caches {"vel": {"tz x ny": ms}, "str": {...}, "pml": {"flat": ms, "tiled": ms}}
-    and, for full-space problems, "fused": {"ny": ms}."""
+    and "fused": {"ny": ms} (the fused sweep, plus the top-plane stresses
+    under a free surface)."""
     case = solver.case
     before = solver.get_tuning()
     table = {"vel": {}, "str": {}, "pml": {}}
-    if case.fspace:
-        nyi = case.nyt - 2 - case.nabc
-        table["fused"] = {}
-        for ny in FUSED_NYS:
-            if ny > max(32, 2 * nyi):
-                continue
-            solver.set_tuning(5, 16, ny, 16, ny)
-            solver.step(warmup, outputs=False)
-            solver.set_profiling(True)
-            solver.step(steps, outputs=False)
-            kt = solver.kernel_times()
-            solver.set_profiling(False)
-            table["fused"][str(ny)] = kt["fused_interior"][0] / kt["fused_interior"][1]
+    nyi = case.nyt - 2 - case.nabc
+    table["fused"] = {}
+    for ny in FUSED_NYS:
+        if ny > max(32, 2 * nyi):
+            continue
+        solver.set_tuning(5, 16, ny, 16, ny)
+        solver.step(warmup, outputs=False)
+        solver.set_profiling(True)
+        solver.step(steps, outputs=False)
+        kt = solver.kernel_times()
+        solver.set_profiling(False)
+        ms = kt["fused_interior"][0] / kt["fused_interior"][1]
+        if "str_top" in kt:          # free surface: the top planes' stresses
+            ms += kt["str_top"][0] / kt["str_top"][1]
+        table["fused"][str(ny)] = ms
     for wide, tz, ny in candidates(case):
         for pml_mode in ((1, 3) if (wide, tz, ny) == (False, 16, 32) else (1,)):
             solver.set_tuning(pml_mode | (24 if wide else 0), tz, ny, tz, ny)
@@ -154,17 +156,27 @@ def lookup(case: C.Case) -> Optional[Dict]:
 FUSED_MARGIN = 0.95
 
 
-def fused_ny(case: C.Case) -> int:
-    """Rows per block of the fused sweep from the problem's own geometry:
-    y chunks for about 1024 workgroups (4 per CU), at least 32 rows.  The
-    fused sweep runs one workgroup per CU at a time, so its time follows
-    the block count of the exact grid; a row count measured on another
-    member of the size class does not carry over (a 248x274x310 grid at the
-    class's 128 rows made 348 blocks and ran 16% slower than separate)."""
-    tiles = -(-case.nxtT // 64) * -(-case.nztT // 11)
-    nyi = case.nyt - 2 - case.nabc
-    fgy = -(-1024 // max(tiles, 1))
-    return max(32, -(-nyi // fgy))
+def fused_ny(case: C.Case, ncu: int = 256) -> int:
+    """Rows per block of the fused sweep from the problem's own geometry.
+    The sweep holds one workgroup per CU for the whole of its y chunk, and
+    each chunk re-reads a 4-row ring, so its time goes as waves x (rows +
+    4) with waves = ceil(blocks / CUs); the row count minimising that is
+    taken (the same rule as the library's default).  A row count measured
+    on another member of the size class does not carry over: a 248x274x310
+    grid at the class's 128 rows made 348 blocks, a second wave a third
+    full, and ran 16% slower than the separate sweeps.  The model orders
+    the measured row counts of the 1024x512x512 and 512x256x256 grids as
+    measured, to within 0.3%."""
+    tiles = max(1, -(-case.nxtT // 64) * -(-case.nztT // 11))
+    nyi = max(1, case.nyt - 2 - case.nabc)
+    best = None
+    for fgy in range(1, 33):
+        rows = -(-nyi // fgy)
+        chunks = -(-nyi // rows)
+        cost = -(-(tiles * chunks) // ncu) * (rows + 4)
+        if best is None or cost < best[0]:
+            best = (cost, rows)
+    return best[1]
 
 
 def choose(case: C.Case, tz: Optional[int] = None) -> Optional[Tuple[int, int, int, int, int]]:
@@ -184,7 +196,7 @@ def choose(case: C.Case, tz: Optional[int] = None) -> Optional[Tuple[int, int, i
     # (rows per block from the problem's geometry, fused_ny) when its best
     # measured time beats the best separate pair by FUSED_MARGIN
     fz = e.get("fused")
-    if tz is None and fz and case.fspace:
+    if tz is None and fz:
         fms = min(fz.values())
         if fms < FUSED_MARGIN * (_best(e["vel"])[3] + _best(e["str"])[3]):
             return (mode | 4, tv, fused_ny(case), ts, ns)

@@ -222,8 +222,9 @@ __global__ void __launch_bounds__(fused::NT) k_batch_fused(BatchArgs A, int seg_
   const int sg = find_seg(A.seg, seg_lo, seg_hi, item);
   const BatchDomain &D = A.dom[A.seg[sg].dom];
   const int b = item - A.seg[sg].start;
+  // members run the fused stage in full space only (describe)
   fused::fused_body(D.p.g, D.p.c, D.p.f[0], D.p.f[1], D.p.ctl, b % D.fgx, (b / D.fgx) % D.fgz, b / (D.fgx * D.fgz),
-                    D.nyf, L);
+                    D.nyf, D.p.g.nzt - D.p.g.nfs, L);
 }
 
 // One stage of every member; the stage is a template argument, so each
@@ -365,6 +366,7 @@ BatchDomain describe(const fd3d_handle *h, int tz, bool allow_fused) {
   std::vector<KernelId> seq;
   for (KernelId id : h->seq) {
     if (id == K_FUSED && !fz) { seq.push_back(K_VEL_INTERIOR); continue; }
+    if (id == K_STR_TOP) continue;   // free-surface members run the separate sweeps here
     seq.push_back(id);
   }
   if (!fz) {

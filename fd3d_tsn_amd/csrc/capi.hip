@@ -51,17 +51,31 @@ Dims make_dims(const fd3d_config &c) {
 }
 
 // Default launch shape: tall x-z tiles where the grid allows, and enough
-// y chunks for about 2048 blocks (8 per CU) on large grids.  Full-space
-// grids with at least kFusedTileRows tile-rows of the fused sweep (its x-z
+// y chunks for about 2048 blocks (8 per CU) on large grids.  Grids with at
+// least kFusedTileRows tile-rows of the fused sweep (its x-z
 // tiles of 64 x 11 cells times the interior rows) run the fused interior
-// sweep, in y chunks of at least 32 rows for about 1024 blocks.  Measured
+// sweep.  Its y chunks minimise waves x (rows + 4), waves = ceil(blocks /
+// CUs): each workgroup holds a CU for its whole chunk and re-reads a 4-row
+// ring (fused_rows; the autotuner uses the same rule).  Measured
 // (interior sweeps, fused against separate): 1024x512x512 4.85 against
 // 5.85 ms, 256x256x256 0.36 against 0.45, 128x256x256 0.21 against 0.25;
 // even at 256x128x128 (6048 tile-rows); slower below, 192x128x128 0.107
 // against 0.086, 128x64x64 0.067 against 0.032.
 constexpr long long kFusedTileRows = 8192;
 
-Tuning default_tuning(const Dims &g) {
+int fused_rows(long long tiles, int nyi, int ncu) {
+  long long best = -1;
+  int brows = nyi;
+  for (int fgy = 1; fgy <= 32; ++fgy) {
+    const int rows = (nyi + fgy - 1) / fgy;
+    const long long chunks = (nyi + rows - 1) / rows;
+    const long long cost = ((tiles * chunks + ncu - 1) / ncu) * (rows + 4);
+    if (best < 0 || cost < best) { best = cost; brows = rows; }
+  }
+  return brows;
+}
+
+Tuning default_tuning(const Dims &g, int ncu) {
   Tuning t{};
   t.mode = 1;
   const int nx = g.nxt - 2 * g.nabc, nz = g.nzt - g.nfs - g.nabc, ny = g.nyt - 2 - g.nabc;
@@ -74,12 +88,9 @@ Tuning default_tuning(const Dims &g) {
   t.tz_vel = t.tz_str = tz;
   t.ny_vel = t.ny_str = rows;
   const long long ftiles = (long long)((nx + 63) / 64) * ((nz + 10) / 11);
-  if (g.fspace && ny > 0 && ftiles * ny >= kFusedTileRows) {
-    long long fgy = (1024 + ftiles - 1) / ftiles;
-    int frows = (int)((ny + fgy - 1) / fgy);
-    if (frows < 32) frows = 32;
+  if (ny > 0 && ftiles * ny >= kFusedTileRows) {
     t.mode = 1 | 4;
-    t.ny_vel = frows;
+    t.ny_vel = fused_rows(ftiles, ny, ncu > 0 ? ncu : 256);
   }
   return t;
 }
@@ -343,10 +354,13 @@ void drop_graph(fd3d_handle *h) {
 void build_seq(fd3d_handle *h) {
   drop_graph(h);
   const Problem &p = h->p;
-  const bool fused = (p.tune.mode & 4) && p.g.fspace;
+  const bool fused = p.tune.mode & 4;
   h->seq.clear();
   if (fused) {
+    // under a free surface the ghost velocities come after the fused sweep
+    // and before the stresses of the top two planes (str_top)
     h->seq = {K_VEL_FAULT, K_PML_VEL, K_FUSED};
+    if (!p.g.fspace) { h->seq.push_back(K_FUVW_UV); h->seq.push_back(K_FUVW_W); h->seq.push_back(K_STR_TOP); }
   } else {
     h->seq = {K_VEL_INTERIOR, K_VEL_FAULT, K_PML_VEL};
     if (!p.g.fspace) { h->seq.push_back(K_FUVW_UV); h->seq.push_back(K_FUVW_W); }
@@ -404,7 +418,13 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
   Problem &p = h->p;
   p.g = make_dims(*cfg);
   p.c = make_coef(*cfg);
-  p.tune = default_tuning(p.g);
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      ncu = 0;
+    (void)hipGetLastError();
+    p.tune = default_tuning(p.g, ncu);
+  }
   long long nsplit = 0, nprof = 0;
   if (build_regions(*cfg, p.g, p.pml, nsplit, nprof) != 0) {
     delete h;
@@ -892,8 +912,6 @@ int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str
   if ((mode & 24) && !(mode & 1)) return fail(h, FD3D_E_INVALID, "tile widths (bits 3, 4) need bit 0");
   if (((mode & 8) && tz_vel > 8) || ((mode & 16) && tz_str > 8))
     return fail(h, FD3D_E_INVALID, "128-wide tiles take tz 4 or 8");
-  if ((mode & 4) && !h->p.g.fspace)
-    return fail(h, FD3D_E_INVALID, "the fused interior sweep needs a full-space problem");
   if ((mode & 4) && ny_vel < 1) return fail(h, FD3D_E_INVALID, "rows per block must be >= 1");
   if ((mode & 1) && (!ok_tz(tz_vel) || !ok_tz(tz_str) || ny_vel < 1 || ny_str < 1))
     return fail(h, FD3D_E_INVALID, "tile height must be 4, 8 or 16 and rows >= 1");

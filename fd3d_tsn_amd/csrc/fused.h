@@ -36,8 +36,10 @@
 //
 // Per-point arithmetic is that of tiles.h (the same operands, the same
 // association), so the result is bit-identical to the separate sweeps.
-// Full-space problems only (with a free surface the ghost velocities of
-// fuvw must be formed between the two updates).
+// Under a free surface the stresses of the top two interior planes need
+// the ghost velocities that fuvw forms from the new velocities: the sweep
+// stores stresses up to kmax_s = ks - 2 only (ks = nzt - nfs), and
+// str_top does the last two planes after fuvw.
 #pragma once
 #include "stages.h"
 
@@ -120,7 +122,8 @@ struct FusedLds {
 // by of NY rows.  Lanes are threadIdx.x (0..63) of wavefront threadIdx.y
 // (0..15) of a 64 x 16 block.
 __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const Fields &f0, const Fields &f1,
-                                           const Ctl *ctl, int bx, int bz, int by, int NY, FusedLds &L) {
+                                           const Ctl *ctl, int bx, int bz, int by, int NY, int kmax_s,
+                                           FusedLds &L) {
   float(*Sb)[NPR * PW] = L.S;
   float(*Vb)[3][NZP * PW] = L.V;
   // the old set (fo) is set it & 1, the new one the other (see field_pair);
@@ -145,6 +148,7 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
   const bool in_xz = has && i >= 1 && i <= g.nxt && k >= 1 && k <= g.nzt;
   const bool int_xz = has && i >= g.nabc + 1 && i <= g.nxt - g.nabc && k >= g.nabc + 1 && k <= g.nzt - g.nfs;
   const bool core = int_xz && wv >= 2 && wv < 2 + ZC;
+  const bool core_s = core && k <= kmax_s;   // stores new stresses
   const int t = (z + 2) * PW + PX0 + x;   // the lane's slot in a velocity plane
   const int sy = g.sy;
   const float c1 = c.c1, c2 = c.c2, dth = c.dth;
@@ -274,7 +278,7 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
       }                                                                             \
     }                                                                               \
     const int s = r - 2;                                                            \
-    if (core && s >= j0 && s <= je) {                                               \
+    if (core_s && s >= j0 && s <= je) {                                             \
       const float *Lu = V[0], *Lv = V[1], *Lw = V[2];                               \
       const float u0 = uB, v0 = vC, w0 = wB;                                        \
       const float dux1 = c1 * (Lu[t + 1] - u0), dux2 = c2 * (Lu[t + 2] - Lu[t - 1]); \

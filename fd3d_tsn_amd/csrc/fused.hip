@@ -10,7 +10,8 @@ __global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fiel
                                                        int gz, int gy, int NY) {
   __shared__ FusedLds L;
   const int lid = xcd_remap(blockIdx.x, gx * gz * gy);
-  fused_body(g, c, f0, f1, ctl, lid % gx, (lid / gx) % gz, lid / (gx * gz), NY, L);
+  const int kmax_s = g.nzt - g.nfs - (g.fspace ? 0 : 2);
+  fused_body(g, c, f0, f1, ctl, lid % gx, (lid / gx) % gz, lid / (gx * gz), NY, kmax_s, L);
 }
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }

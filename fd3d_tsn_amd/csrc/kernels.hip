@@ -16,7 +16,7 @@
 namespace fd3d {
 
 const char *kKernelNames[K_COUNT] = {
-    "vel_interior", "vel_fault", "pml_vel", "fuvw_uv", "fuvw_w", "fused_interior",
+    "vel_interior", "vel_fault", "pml_vel", "fuvw_uv", "fuvw_w", "fused_interior", "str_top",
     "str_interior", "str_fault", "pml_str", "fres",
     "traction", "traction_mirror", "tabs", "friction", "outputs", "rates", "nucleation", "advance"};
 
@@ -53,6 +53,13 @@ __global__ void __launch_bounds__(256) k_fuvw_w(Dims g, Fields f0, Fields f1, co
 __global__ void __launch_bounds__(256) k_str_interior(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl) {
   PAIR;
   str_interior_body(g, c, fo, fn, blockIdx.x, blockIdx.y, blockIdx.z, threadIdx.x, threadIdx.y);
+}
+// Interior stresses of the planes k0 .. k0 + gridDim.z - 1 (the top two
+// interior planes under a free surface, which need the ghost velocities of
+// fuvw: the fused sweep leaves them to this kernel).
+__global__ void __launch_bounds__(256) k_str_top(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl, int gbz0) {
+  PAIR;
+  str_interior_body(g, c, fo, fn, blockIdx.x, blockIdx.y, gbz0 + (int)blockIdx.z, threadIdx.x, threadIdx.y);
 }
 __global__ void __launch_bounds__(256) k_str_fault(Dims g, Coef c, Fields f0, Fields f1, Fault fl, const Ctl *ctl) {
   PAIR;
@@ -198,6 +205,15 @@ hipError_t launch(KernelId id, const Problem &p, hipStream_t s) {
       break;
     case K_FUSED:
       return launch_fused(p, s);
+    case K_STR_TOP: {
+      // planes max(nabc+1, ks-1) .. ks, ks = nzt - nfs: the ones the fused
+      // sweep does not store under a free surface
+      if (nyi <= 0 || nk <= 0) return hipSuccess;
+      const int gbz0 = nk >= 2 ? nk - 2 : 0;
+      hipLaunchKernelGGL(k_str_top, dim3(cdiv(ni, BX), cdiv(nyi, BY), nk - gbz0), b2, 0, s, g, p.c, p.f[0], p.f[1],
+                         p.ctl, gbz0);
+      break;
+    }
     case K_RATES:
       if (!p.rates.on) return hipSuccess;
       hipLaunchKernelGGL(k_rates, dim3(1 + cdiv(p.rates.NL * p.rates.NW, 1024)), dim3(1024), 0, s, g, p.c, p.f[0],

@@ -177,11 +177,13 @@ int fd3d_download_source_bins(fd3d_handle *h, float *msrx, float *msrz, float *m
  * block owns a 64 x tz tile of the x-z plane and marches ny rows along y
  * (tz in {4, 8, 16}, ny >= 1), else one thread per cell; mode bit 1: PML
  * sweeps as 64 x 8 x-z tiles marching y, else one thread per cell; mode
- * bit 2: one fused velocity + stress pass over the interior (full space
- * only, ny_vel rows per block); mode bits 3 and 4 (with bit 0): velocity
- * and stress tiles 128 wide instead of 64 (tz in {4, 8}).  fd3d_create picks a default from
- * the grid size (the fused sweep for full-space grids whose 64 x 11 x-z
- * tiles times interior rows reach 8192, such as 256x256x256 and up);
+ * bit 2: one fused velocity + stress pass over the interior (ny_vel rows
+ * per block; under a free surface the top two interior planes get their
+ * stresses after the ghost velocities, in a small pass of their own); mode
+ * bits 3 and 4 (with bit 0): velocity and stress tiles 128 wide instead of
+ * 64 (tz in {4, 8}).  fd3d_create picks a default from the grid size (the
+ * fused sweep for grids whose 64 x 11 x-z tiles times interior rows reach
+ * 8192, such as 256x256x256 and up);
  * fd3d_tsn_amd/autotune.py measures and caches the best.
  * Results do not depend on the shape (bit-identical). */
 int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str);

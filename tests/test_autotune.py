@@ -58,9 +58,9 @@ def test_choose_takes_fused_sweep_when_faster(cache):
            "pml": {"flat": 3.0}, "fused": {"64": 5.0, "256": 4.8}}
     json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
     # 4.8 < 0.95 (2.5 + 4.0): fused (bit 2), rows per block from the grid
-    # (752 fused tiles, 510 rows: 2 chunks of 255)
-    assert A.fused_ny(case) == 255
-    assert A.choose(case) == (1 | 8 | 4, 8, 255, 16, 32)
+    # (752 fused tiles, 510 rows: one chunk, 3 waves x 514 rows)
+    assert A.fused_ny(case) == 510
+    assert A.choose(case) == (1 | 8 | 4, 8, 510, 16, 32)
     # the batch engine runs separate sweeps only
     assert A.choose(case, tz=16) == (1, 16, 32, 16, 32)
     tab["fused"] = {"64": 6.3}          # within the 5% margin of 6.5: separate
@@ -68,20 +68,23 @@ def test_choose_takes_fused_sweep_when_faster(cache):
     assert A.choose(case) == (1 | 8, 8, 32, 16, 32)
 
 
-def test_fused_ny_targets_about_1024_blocks():
+def test_fused_ny_minimises_waves_times_rows():
+    # 512x256x256: 192 tiles, 254 rows.  Measured fused sweep: 64 rows 0.667 ms,
+    # 32 rows 0.693, 43 rows 0.718, 256 rows 0.836, 128 rows 0.848 - the order
+    # of waves x (rows + 4): 204, 216, 235, 258, 264
+    c = C.c3_fvw()
+    assert A.fused_ny(c) == 64
     c = C.c2_large_sw().replace(nxtT=248, nytT=274, nztT=310)   # 116 tiles, 272 rows
-    assert A.fused_ny(c) == 32                                  # 9 chunks: 1044 blocks
-    c = C.c2_large_sw().replace(nxtT=256, nytT=256, nztT=256)   # 96 tiles, 254 rows
-    assert A.fused_ny(c) == 32
+    assert A.fused_ny(c) == 136                                 # 2 chunks: 232 blocks, one wave
 
 
-def test_fused_never_chosen_with_free_surface(cache):
-    case = C.c1_plain_sw()
-    if case.fspace:
-        case = case.replace(fspace=False)
+def test_fused_chosen_with_free_surface_alone_not_in_batches(cache):
+    case = C.c3_fvw()            # free surface
     tab = {"vel": {"16x32": 3.0}, "str": {"16x32": 4.0}, "pml": {}, "fused": {"64": 1.0}}
     json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
-    assert A.choose(case)[0] & 4 == 0
+    assert A.choose(case)[0] & 4
+    # the batch engine runs free-surface members with the separate sweeps
+    assert A.batch_member(case, A.choose(case, 16), 16) == A.choose(case, 16)
 
 
 def test_batch_member_fused_chunks():

@@ -195,20 +195,26 @@ def test_wide_tiles_reject_bad_shapes():
             assert e.value.code == -1, bad
 
 
-FUSED = [("fs_ny3", dict(nabc=5, n=(70, 17, 21), seed=33), 3),
-         ("fs_ny1", dict(nabc=4, n=(37, 13, 30), seed=34), 1),
-         ("fs_ny64", dict(nabc=6, n=(130, 20, 14), seed=35), 64),
-         ("fs_tallz", dict(nabc=4, n=(24, 10, 50), seed=36), 4)]
+FUSED = [("fs_ny3", dict(nabc=5, n=(70, 17, 21), seed=33), 3, True),
+         ("fs_ny1", dict(nabc=4, n=(37, 13, 30), seed=34), 1, True),
+         ("fs_ny64", dict(nabc=6, n=(130, 20, 14), seed=35), 64, True),
+         ("fs_tallz", dict(nabc=4, n=(24, 10, 50), seed=36), 4, True),
+         ("free_ny3", dict(nabc=5, n=(70, 17, 21), seed=37), 3, False),
+         ("free_ny64", dict(nabc=6, n=(130, 20, 14), seed=38), 64, False),
+         ("free_tallz", dict(nabc=4, n=(24, 10, 50), seed=39), 4, False),
+         ("free_nz5", dict(nabc=4, n=(40, 12, 5), seed=40), 8, False)]
 
 
-@pytest.mark.parametrize("name,kw,ny", FUSED, ids=[f[0] for f in FUSED])
-def test_fused_interior_bit_exact(name, kw, ny):
-    """The fused velocity + stress sweep (full space) gives the oracle's
-    wavefield and outputs bit for bit: partial tiles in x and z, chunks of
-    1 row up to the whole y range, the near-fault rows and PML neighbours
-    read from the other kernels' new velocities.  Starts from a random
-    wavefield so every cell is non-zero from the first step."""
-    case = C.tiny(friction="sw", fspace=True, ntfd=12, **kw)
+@pytest.mark.parametrize("name,kw,ny,fspace", FUSED, ids=[f[0] for f in FUSED])
+def test_fused_interior_bit_exact(name, kw, ny, fspace):
+    """The fused velocity + stress sweep gives the oracle's wavefield and
+    outputs bit for bit: partial tiles in x and z, chunks of 1 row up to the
+    whole y range, the near-fault rows and PML neighbours read from the
+    other kernels' new velocities; under a free surface the top two planes'
+    stresses come after the ghost velocities (down to 2 interior planes).
+    Starts from a random wavefield so every cell is non-zero from the first
+    step."""
+    case = C.tiny(friction="sw", fspace=fspace, ntfd=12, **kw)
     say(f"[parity] fused sweep {name} ny={ny} grid {case.nxt}x{case.nyt}x{case.nzt}")
     rng = np.random.Generator(np.random.PCG64(7))
     init = {n: (rng.standard_normal((case.nxt, case.nyt, case.nzt)) *
@@ -232,11 +238,31 @@ def test_fused_interior_bit_exact(name, kw, ny):
         assert bad.size == 0, f"field {n}: {len(bad)} mismatches, first (0-based) {bad[0].tolist()}"
 
 
-def test_fused_rejected_with_free_surface():
-    case = C.tiny(friction="sw", fspace=False, ntfd=2, seed=37)
-    with capi.Solver(case) as s:
-        with pytest.raises(capi.FD3DError):
-            s.set_tuning(5, 16, 8, 16, 8)
+@pytest.mark.parametrize("fspace", [False, True], ids=["free", "fs"])
+def test_fused_equals_separate_rate_and_state(fspace):
+    """Rate-and-state from a random wavefield: the fused sweep (graph
+    replay) and the separate sweeps (kernel by kernel) agree bit for bit in
+    every field and output, free surface and full space."""
+    case = C.tiny(friction="fvw", fspace=fspace, ntfd=16, nabc=5, n=(70, 17, 24), seed=57)
+    say(f"[parity] fused against separate, rate-and-state, fspace={fspace}")
+    rng = np.random.Generator(np.random.PCG64(9))
+    init = {n: (rng.standard_normal((case.nxt, case.nyt, case.nzt)) *
+                (1e3 if n in ("xx", "yy", "zz", "xy", "yz", "xz") else 1e-6)).astype(np.float32)
+            for n in capi.FIELDS}
+    res = []
+    for tuning, prof in (((5, 16, 4, 16, 4), False), ((1, 8, 16, 8, 16), True)):
+        with capi.Solver(case) as s:
+            for n, a in init.items():
+                s.set_field(n, a)
+            s.set_tuning(*tuning)
+            s.set_profiling(prof)
+            out = s.step(case.ntfd)
+            res.append((out, {n: s.field(n) for n in capi.FIELDS}))
+    (oa, fa), (ob, fb) = res
+    for n in OUTS + ("psi",):
+        assert np.array_equal(oa[n], ob[n]), n
+    for n in capi.FIELDS:
+        assert np.array_equal(fa[n], fb[n]), n
 
 
 def layered_medium(case, layers=((5200.0, 3000.0, 2500.0), (5600.0, 3230.0, 2600.0), (6000.0, 3464.0, 2670.0))):
