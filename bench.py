@@ -176,6 +176,7 @@ def bench_batched(a):
         s.sync()
         plain.append((time.perf_counter() - t1) / a.steps)
     # bytes each member moves per step in the batch (fused members: 72 B per bulk cell)
+    # (the batch engine runs the fused stage for full-space members only)
     step_b = [((MOVED_BULK_FUSED if p is not None and p[0] & 4 and c.fspace else MOVED_BULK) * nb + MOVED_PML * npml)
               for c, p, (_, nb, npml) in zip(cases, plan, (s.step_bytes() for s in solvers))]
     step_plain = [MOVED_BULK * nb + MOVED_PML * npml for _, nb, npml in (s.step_bytes() for s in solvers)]
@@ -318,7 +319,7 @@ def main():
     s.set_profiling(False)
     step_bytes, nbulk, npml = s.step_bytes()
     tuning = s.get_tuning()
-    fused = bool(tuning[0] & 4) and case.fspace
+    fused = bool(tuning[0] & 4)
     moved = (MOVED_BULK_FUSED if fused else MOVED_BULK) * nbulk + MOVED_PML * npml
     s.close()
 
