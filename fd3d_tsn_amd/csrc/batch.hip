@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "handle.h"
@@ -244,28 +245,21 @@ void launch_stage(int n, hipStream_t st, const BatchArgs &A, int lo, int hi, int
   hipLaunchKernelGGL((k_batch_stage<TZ, FVW, S>), dim3(n), dim3(64, TZ), 0, st, A, lo, hi, i0);
 }
 
+// Stage launch by run-time stage id: a table of the per-stage
+// instantiations over every KernelId (0 .. K_COUNT-1), so it follows the
+// enum as stages are added.
+template <int TZ, bool FVW, int... S>
+void launch_stage_tab(int s, int n, hipStream_t st, const BatchArgs &A, int lo, int hi, int i0,
+                      std::integer_sequence<int, S...>) {
+  using Fn = void (*)(int, hipStream_t, const BatchArgs &, int, int, int);
+  static constexpr Fn tab[] = {&launch_stage<TZ, FVW, S>...};
+  static_assert(sizeof(tab) / sizeof(tab[0]) == K_COUNT, "one launcher per stage");
+  if (s >= 0 && s < K_COUNT) tab[s](n, st, A, lo, hi, i0);
+}
+
 template <int TZ, bool FVW>
 void launch_stage_rt(int s, int n, hipStream_t st, const BatchArgs &A, int lo, int hi, int i0) {
-  switch (s) {
-    case 0: launch_stage<TZ, FVW, 0>(n, st, A, lo, hi, i0); break;
-    case 1: launch_stage<TZ, FVW, 1>(n, st, A, lo, hi, i0); break;
-    case 2: launch_stage<TZ, FVW, 2>(n, st, A, lo, hi, i0); break;
-    case 3: launch_stage<TZ, FVW, 3>(n, st, A, lo, hi, i0); break;
-    case 4: launch_stage<TZ, FVW, 4>(n, st, A, lo, hi, i0); break;
-    case 5: launch_stage<TZ, FVW, 5>(n, st, A, lo, hi, i0); break;
-    case 6: launch_stage<TZ, FVW, 6>(n, st, A, lo, hi, i0); break;
-    case 7: launch_stage<TZ, FVW, 7>(n, st, A, lo, hi, i0); break;
-    case 8: launch_stage<TZ, FVW, 8>(n, st, A, lo, hi, i0); break;
-    case 9: launch_stage<TZ, FVW, 9>(n, st, A, lo, hi, i0); break;
-    case 10: launch_stage<TZ, FVW, 10>(n, st, A, lo, hi, i0); break;
-    case 11: launch_stage<TZ, FVW, 11>(n, st, A, lo, hi, i0); break;
-    case 12: launch_stage<TZ, FVW, 12>(n, st, A, lo, hi, i0); break;
-    case 13: launch_stage<TZ, FVW, 13>(n, st, A, lo, hi, i0); break;
-    case 14: launch_stage<TZ, FVW, 14>(n, st, A, lo, hi, i0); break;
-    case 15: launch_stage<TZ, FVW, 15>(n, st, A, lo, hi, i0); break;
-    case 16: launch_stage<TZ, FVW, 16>(n, st, A, lo, hi, i0); break;
-    default: launch_stage<TZ, FVW, 17>(n, st, A, lo, hi, i0); break;
-  }
+  launch_stage_tab<TZ, FVW>(s, n, st, A, lo, hi, i0, std::make_integer_sequence<int, K_COUNT>{});
 }
 
 // Per-step reset of the queue head and the done counters (a kernel, so
