@@ -170,10 +170,11 @@ struct Outputs {
 
 // Launch shapes of the interior and PML sweeps; chosen per grid size by
 // the autotuner.  mode bit 0: tiled interior sweeps (interior.hip) instead
-// of one thread per cell; bit 1: z-march PML sweeps (pml.hip) instead of
+// of one thread per cell; bit 1: tiled PML sweeps (pml.hip) instead of
 // one thread per cell (kernels.hip); bit 2: the fused interior sweep
-// (fused.hip, full space only; rows per block from ny_vel); bits 3 and 4:
-// velocity and stress tiles 128 cells wide along x instead of 64.
+// (fused.h; rows per block from ny_vel; under a free surface followed by
+// the top-plane stresses of k_str_top); bits 3 and 4: velocity and stress
+// tiles 128 cells wide along x instead of 64.
 struct Tuning {
   int mode;
   int tz_vel, ny_vel;   // velocity: x-z tile height (4, 8, 16), rows per block along y
