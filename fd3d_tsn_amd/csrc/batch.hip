@@ -325,11 +325,12 @@ BatchDomain describe(const fd3d_handle *h, int tz, bool allow_fused) {
   const int ni = g.nxt - 2 * g.nabc, nyi = g.nyt - 2 - g.nabc, nk = g.nzt - g.nfs - g.nabc;
   D.gx = cdiv(ni, 64);
   D.gz = cdiv(nk, tz);
+  const bool fz = allow_fused && (h->p.tune.mode & 4) && g.fspace && nyi > 0;
   D.nyv = std::max(1, h->p.tune.ny_vel);
   D.nys = std::max(1, h->p.tune.ny_str);
   // a member that asks for the fused sweep holds its fused rows in ny_vel;
-  // when the batch runs the separate sweeps instead, both use ny_str
-  if ((h->p.tune.mode & 4) && g.fspace) D.nyv = D.nys;
+  // when it runs the separate sweeps here instead, both use ny_str
+  if ((h->p.tune.mode & 4) && !fz) D.nyv = D.nys;
   D.gyv = nyi > 0 ? cdiv(nyi, D.nyv) : 0;
   const int gys = nyi > 0 ? cdiv(nyi, D.nys) : 0;
   D.fpx = cdiv(g.nxt, 64);
@@ -337,7 +338,6 @@ BatchDomain describe(const fd3d_handle *h, int tz, bool allow_fused) {
   const int nxn = g.nxt - g.nabc - (g.nabc - 1) + 1, nzn = g.nzt - g.nfs - (g.nabc - 1) + 1;
   D.nux = cdiv(nxn, 64);
   D.nout = D.fix * cdiv(nk, tz);
-  const bool fz = allow_fused && (h->p.tune.mode & 4) && g.fspace && nyi > 0;
   D.fgx = cdiv(ni, fused::LX);
   D.fgz = cdiv(nk, fused::ZC);
   D.nyf = std::max(1, h->p.tune.ny_vel);
