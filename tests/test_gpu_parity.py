@@ -202,11 +202,14 @@ FUSED = [("fs_ny3", dict(nabc=5, n=(70, 17, 21), seed=33), 3, True),
          ("free_ny3", dict(nabc=5, n=(70, 17, 21), seed=37), 3, False),
          ("free_ny64", dict(nabc=6, n=(130, 20, 14), seed=38), 64, False),
          ("free_tallz", dict(nabc=4, n=(24, 10, 50), seed=39), 4, False),
-         ("free_nz5", dict(nabc=4, n=(40, 12, 5), seed=40), 8, False)]
+         ("free_nz5", dict(nabc=4, n=(40, 12, 5), seed=40), 8, False),
+         ("fs_tiledpml", dict(nabc=8, n=(70, 17, 21), seed=41), 8, True, 7),
+         ("free_tiledpml", dict(nabc=8, n=(70, 17, 21), seed=42), 8, False, 7)]
 
 
-@pytest.mark.parametrize("name,kw,ny,fspace", FUSED, ids=[f[0] for f in FUSED])
-def test_fused_interior_bit_exact(name, kw, ny, fspace):
+@pytest.mark.parametrize("name,kw,ny,fspace,mode", [f if len(f) == 5 else f + (5,) for f in FUSED],
+                         ids=[f[0] for f in FUSED])
+def test_fused_interior_bit_exact(name, kw, ny, fspace, mode):
     """The fused velocity + stress sweep gives the oracle's wavefield and
     outputs bit for bit: partial tiles in x and z, chunks of 1 row up to the
     whole y range, the near-fault rows and PML neighbours read from the
@@ -227,8 +230,8 @@ def test_fused_interior_bit_exact(name, kw, ny, fspace):
     with capi.Solver(case) as s:
         for n, a in init.items():
             s.set_field(n, a)
-        s.set_tuning(5, 16, ny, 16, ny)
-        assert s.get_tuning()[0] == 5
+        s.set_tuning(mode, 16, ny, 16, ny)     # mode 7: with the tiled PML sweeps
+        assert s.get_tuning()[0] == mode
         rg = s.step(case.ntfd)
         fields = {n: s.field(n) for n in capi.FIELDS}
     for n in OUTS:
