@@ -133,7 +133,9 @@ class Server:
         self.tz = tz
         self.batch = capi.Batch(max_slots, tz=tz, mode=mode)
         self.log = log or (lambda m: None)
-        self.gpu_busy_s = 0.0
+        self.gpu_busy_s = 0.0      # host time spent waiting for chunks
+        self.gpu_step_s = 0.0      # GPU time of the batch steps (HIP events)
+        self.chunks = 0
 
     def close(self):
         self.batch.close()
@@ -178,6 +180,8 @@ class Server:
                 t1 = clock()
                 self.batch.sync()
                 self.gpu_busy_s += clock() - t1
+                self.gpu_step_s += self.batch.elapsed()[0] * 1e-3
+                self.chunks += 1
                 done = []
                 for a in active:
                     a["left"] -= n
