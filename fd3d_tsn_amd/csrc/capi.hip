@@ -690,29 +690,33 @@ int fd3d_init_state(fd3d_handle *h) {
 // Kernels of one half-step that read only the old set (velocities) or
 // only the new velocities and old stresses (stresses) and write disjoint
 // cells: interior, fault-plane and PML updates.
-static int half_step_group(KernelId id) {
+static int half_step_group(KernelId id, bool small) {
   switch (id) {
-    case K_VEL_INTERIOR: case K_VEL_FAULT: case K_PML_VEL: return 1;
-    case K_STR_INTERIOR: case K_STR_FAULT: case K_PML_STR: return 2;
+    case K_VEL_INTERIOR: return small ? 1 : 0;
+    case K_VEL_FAULT: case K_PML_VEL: return 1;
+    case K_STR_INTERIOR: return small ? 2 : 0;
+    case K_STR_FAULT: case K_PML_STR: return 2;
     default: return 0;
   }
 }
 
-// Problems below this many cells capture overlapping half-steps: their
-// launches are too small to fill the GPU one after another.  Above it the
-// overlap only makes the sweeps compete for HBM (measured: 128x64x64 step
-// 0.107 -> 0.097 ms; 1024x512x512 step 8.75 -> 9.15 ms).
+// Problems below this many cells capture overlapping half-steps, interior
+// sweeps included: their launches are too small to fill the GPU one after
+// another.  Above it the interior sweeps run alone (overlapping them only
+// makes the sweeps compete for HBM: 1024x512x512 step 8.75 -> 9.15 ms),
+// and only the small fault-plane kernels run beside the PML sweeps they
+// are independent of (vel_fault with pml_vel, str_fault with pml_str:
+// disjoint writes, neither reads what the other writes).
 constexpr long long kOverlapCells = 4LL << 20;
 
 // One step of h->seq captured into a graph on h->stream.  Kernel arguments
 // are fixed for the life of the handle (the wavefield set and the time
-// come from the device step counter), so one graph serves every step.  For
-// small problems, in a run of kernels of the same half-step group the
-// first goes on the main stream and the rest on the side stream (fork and
-// join by events), so the small fault-plane and PML launches overlap the
-// interior sweep.
+// come from the device step counter), so one graph serves every step.  In
+// a run of kernels of the same half-step group (half_step_group) the first
+// goes on the main stream and the rest on the side stream (fork and join
+// by events), so they run side by side.
 static int capture_step(fd3d_handle *h) {
-  const bool overlap = (long long)h->cfg.nxt * h->cfg.nyt * h->cfg.nzt < kOverlapCells;
+  const bool small = (long long)h->cfg.nxt * h->cfg.nyt * h->cfg.nzt < kOverlapCells;
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   auto abort_capture = [&](const std::string &what, hipError_t e) {
     hipGraph_t g = nullptr;
@@ -723,8 +727,8 @@ static int capture_step(fd3d_handle *h) {
   const std::vector<KernelId> &q = h->seq;
   for (size_t n = 0; n < q.size();) {
     size_t m = n + 1;
-    const int grp = overlap ? half_step_group(q[n]) : 0;
-    while (grp && m < q.size() && half_step_group(q[m]) == grp) ++m;
+    const int grp = half_step_group(q[n], small);
+    while (grp && m < q.size() && half_step_group(q[m], small) == grp) ++m;
     hipError_t e;
     if (m - n > 1) {
       if ((e = hipEventRecord(h->fork, h->stream)) != hipSuccess) return abort_capture("fork", e);

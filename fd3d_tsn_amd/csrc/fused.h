@@ -1,7 +1,7 @@
-// Fused interior sweep (device body; fused.hip launches it for one
-// problem, batch.hip as a stage of a batch): the velocity update (dvel, fd3d_deriv.f90:24-193)
+// Fused interior sweep: the velocity update (dvel, fd3d_deriv.f90:24-193)
 // and the stress update (dstres, :513-622) of the interior cells in one
-// pass over HBM.
+// pass over HBM.  Device body only: fused.hip launches it for one problem,
+// batch.hip as a stage of a batch.
 //
 // Separately, the two sweeps move 27 words per interior cell (velocity: 6
 // old stresses and 3 old velocities read, 3 new velocities written;
@@ -21,12 +21,13 @@
 //   1. the old-stress planes of row r (loaded into registers during
 //      iteration r - 2) go into one of two LDS buffers, and every lane puts
 //      its new velocity of row r - 2 (from its register window) into one of
-//      two velocity planes; the loads of row r + 2 are issued; one barrier;
-//   2. every lane forms the new velocity
-//      of its position at row r from the stress planes (x and z neighbours)
-//      and its register windows of xy, yy, yz (y neighbours); core lanes
-//      form the new stresses of row r - 2 from the velocity plane (x and z
-//      neighbours) and their velocity windows (y neighbours).
+//      two velocity planes; the loads of row r + 2 are issued (they are in
+//      flight across the barrier); one barrier;
+//   2. every lane forms the new velocity of its position at row r from the
+//      stress planes (x and z neighbours) and its register windows of xy,
+//      yy, yz (y neighbours); core lanes form the new stresses of row r - 2
+//      from the velocity plane (x and z neighbours) and their velocity
+//      windows (y neighbours).
 // Lanes on the ring around the core recompute velocities that the
 // neighbouring tiles own, so no block waits on another.  Positions outside
 // the interior (PML cells, the near-fault rows j >= nyt-1, the outer faces)
