@@ -163,14 +163,14 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
   const int r0 = j0 - 2, r1 = je + 2;
   const int jlo = g.nabc + 1, jhi = g.nyt - 2;  // interior rows
 
-#define F2_SEL(a) const float *o##a = odd ? f1.a : f0.a; float *n##a = odd ? f0.a : f1.a
-  F2_SEL(xx); F2_SEL(yy); F2_SEL(zz); F2_SEL(xy); F2_SEL(yz); F2_SEL(xz); F2_SEL(u); F2_SEL(v); F2_SEL(w);
-#undef F2_SEL
+#define FUSED_SEL(a) const float *o##a = odd ? f1.a : f0.a; float *n##a = odd ? f0.a : f1.a
+  FUSED_SEL(xx); FUSED_SEL(yy); FUSED_SEL(zz); FUSED_SEL(xy); FUSED_SEL(yz); FUSED_SEL(xz); FUSED_SEL(u); FUSED_SEL(v); FUSED_SEL(w);
+#undef FUSED_SEL
   float *nu_ = nu, *nv_ = nv, *nw_ = nw;
-#define F2_COL(f, j) ((in_xz && (j) >= 1 && (j) <= g.nyt) ? ld32(f, qb + ((j) - 1) * sy) : 0.0f)
+#define FUSED_COL(f, j) ((in_xz && (j) >= 1 && (j) <= g.nyt) ? ld32(f, qb + ((j) - 1) * sy) : 0.0f)
 // velocity source of row j: the old set where this kernel updates the
 // position, the new set where another kernel already did, 0 outside
-#define F2_VSRC(j, a, bb, cc)                                                      \
+#define FUSED_VSRC(j, a, bb, cc)                                                      \
   do {                                                                              \
     a = bb = cc = 0.0f;                                                             \
     if (in_xz && (j) >= 1 && (j) <= g.nyt) {                                        \
@@ -202,7 +202,7 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
       psrc[n] = f == 0 ? oxx : f == 1 ? oxy : f == 2 ? oxz : f == 3 ? oyz : ozz;
     }
   }
-#define F2_PLANES(pf, j)                                                            \
+#define FUSED_PLANES(pf, j)                                                            \
   do {                                                                              \
     const bool jok_ = (j) >= 1 && (j) <= g.nyt;                                     \
     const int jo_ = ((j) - 1) * sy;                                                 \
@@ -215,9 +215,9 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
   // yy(r+2), yz(r+1))
   float xym2 = 0, xym1 = 0, xy0 = 0, yym2 = 0, yym1 = 0, yy0 = 0, yyp1 = 0, yzm2 = 0, yzm1 = 0, yz0 = 0;
   if (int_xz) {
-    xym2 = F2_COL(oxy, r0 - 2); xym1 = F2_COL(oxy, r0 - 1); xy0 = F2_COL(oxy, r0);
-    yym2 = F2_COL(oyy, r0 - 2); yym1 = F2_COL(oyy, r0 - 1); yy0 = F2_COL(oyy, r0); yyp1 = F2_COL(oyy, r0 + 1);
-    yzm2 = F2_COL(oyz, r0 - 2); yzm1 = F2_COL(oyz, r0 - 1); yz0 = F2_COL(oyz, r0);
+    xym2 = FUSED_COL(oxy, r0 - 2); xym1 = FUSED_COL(oxy, r0 - 1); xy0 = FUSED_COL(oxy, r0);
+    yym2 = FUSED_COL(oyy, r0 - 2); yym1 = FUSED_COL(oyy, r0 - 1); yy0 = FUSED_COL(oyy, r0); yyp1 = FUSED_COL(oyy, r0 + 1);
+    yzm2 = FUSED_COL(oyz, r0 - 2); yzm1 = FUSED_COL(oyz, r0 - 1); yz0 = FUSED_COL(oyz, r0);
   }
   // new velocities of the lane's column: v rows r-4..r-1 (vA..vD), u and w
   // rows r-3..r-1 (uA..uC, wA..wC)
@@ -229,14 +229,14 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
   // column and the velocity source of row R.
   f4 pfA[NPF], pfB[NPF];
   float qxyA = 0, qyyA = 0, qyzA = 0, quA, qvA, qwA, qxyB = 0, qyyB = 0, qyzB = 0, quB, qvB, qwB;
-#define F2_FETCH(P, R)                                                              \
+#define FUSED_FETCH(P, R)                                                              \
   do {                                                                              \
-    F2_PLANES(pf##P, R);                                                            \
-    if (int_xz) { qxy##P = F2_COL(oxy, (R) + 1); qyy##P = F2_COL(oyy, (R) + 2); qyz##P = F2_COL(oyz, (R) + 1); } \
-    F2_VSRC(R, qu##P, qv##P, qw##P);                                                \
+    FUSED_PLANES(pf##P, R);                                                            \
+    if (int_xz) { qxy##P = FUSED_COL(oxy, (R) + 1); qyy##P = FUSED_COL(oyy, (R) + 2); qyz##P = FUSED_COL(oyz, (R) + 1); } \
+    FUSED_VSRC(R, qu##P, qv##P, qw##P);                                                \
   } while (0)
 // Iteration r with prefetch set P.
-#define F2_ITER(P)                                                                  \
+#define FUSED_ITER(P)                                                                  \
   do {                                                                              \
     const int buf = r & 1;                                                          \
     float *S = Sb[buf];                                                             \
@@ -250,13 +250,13 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
     const float xyp1 = qxy##P, yyp2 = qyy##P, yzp1 = qyz##P;                        \
     const float uo = qu##P, vo = qv##P, wo = qw##P;                                 \
     /* 2. loads of row r+2 into this set, in flight across the barrier */           \
-    if (r + 2 <= r1) F2_FETCH(P, r + 2);                                            \
+    if (r + 2 <= r1) FUSED_FETCH(P, r + 2);                                            \
     __syncthreads();                                                                \
-    F2_COMPUTE();                                                                   \
+    FUSED_COMPUTE();                                                                   \
   } while (0)
 // New velocity of the lane's position at row r, new stresses of row r-2,
 // then the windows advance to row r+1.
-#define F2_COMPUTE()                                                                \
+#define FUSED_COMPUTE()                                                                \
   do {                                                                              \
     float un = uo, vn = vo, wn = wo;                                                \
     const float xx0 = S[at<0>(x, z)], xz0 = S[at<2>(x, z)], zz0 = S[at<4>(x, z)];   \
@@ -305,20 +305,20 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
     yzm2 = yzm1; yzm1 = yz0; yz0 = yzp1;                                            \
   } while (0)
 
-  F2_FETCH(A, r0);
-  if (r0 + 1 <= r1) F2_FETCH(B, r0 + 1);
+  FUSED_FETCH(A, r0);
+  if (r0 + 1 <= r1) FUSED_FETCH(B, r0 + 1);
   for (int r = r0;;) {
-    F2_ITER(A);
+    FUSED_ITER(A);
     if (++r > r1) break;
-    F2_ITER(B);
+    FUSED_ITER(B);
     if (++r > r1) break;
   }
-#undef F2_FETCH
-#undef F2_ITER
-#undef F2_COMPUTE
-#undef F2_COL
-#undef F2_VSRC
-#undef F2_PLANES
+#undef FUSED_FETCH
+#undef FUSED_ITER
+#undef FUSED_COMPUTE
+#undef FUSED_COL
+#undef FUSED_VSRC
+#undef FUSED_PLANES
 }
 
 }  // namespace fused
