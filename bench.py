@@ -177,7 +177,7 @@ def bench_batched(a):
         plain.append((time.perf_counter() - t1) / a.steps)
     # bytes each member moves per step in the batch (fused members: 72 B per bulk cell)
     # (the batch engine runs the fused stage for full-space members only)
-    step_b = [((MOVED_BULK_FUSED if p is not None and p[0] & 4 and c.fspace else MOVED_BULK) * nb + MOVED_PML * npml)
+    step_b = [((MOVED_BULK_FUSED if p is not None and p[0] & 4 else MOVED_BULK) * nb + MOVED_PML * npml)
               for c, p, (_, nb, npml) in zip(cases, plan, (s.step_bytes() for s in solvers))]
     step_plain = [MOVED_BULK * nb + MOVED_PML * npml for _, nb, npml in (s.step_bytes() for s in solvers)]
     for s, p in zip(solvers, plan):
@@ -213,7 +213,7 @@ def bench_batched(a):
            "config": {"workload": f"batched_{len(cases)}_domains_96-320", "domains": len(cases),
                       "cells": cells, "steps_per_rupture": nsteps_run, "mode": a.batch_mode,
                       "tz": tz, "autotuned_members": tuned,
-                      "fused_members": sum(1 for c, p in zip(cases, plan) if p is not None and p[0] & 4 and c.fspace)},
+                      "fused_members": sum(1 for c, p in zip(cases, plan) if p is not None and p[0] & 4)},
            "batch": {"gpu_ms_per_step": gpu_ms / max(n, 1), "Gcell_updates_per_s": cells / wall / 1e9,
                      "hbm_frac": step_bytes / wall / 1e9 / HBM_PEAK_GBS, "info": info},
            "plain_back_to_back": {"ms_per_step_sum": t_plain * 1e3,

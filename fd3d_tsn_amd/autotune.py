@@ -217,14 +217,14 @@ FUSED_BATCH_NY = 128
 def batch_member(case: C.Case, t: Optional[Tuple[int, int, int, int, int]], tz: int,
                  fused: bool = True) -> Optional[Tuple[int, int, int, int, int]]:
     """A member's shape in a batch: t (from choose at the batch's tz) and,
-    for full-space members when fused is set, the fused interior sweep
-    (mode bit 2) in y chunks of at most FUSED_BATCH_NY rows (in ny_vel).
+    when fused is set, the fused interior sweep (mode bit 2) in y chunks of
+    at most FUSED_BATCH_NY rows (in ny_vel).
     In a batch the fused tiles of all members fill the GPU together, so it
     pays even for grids too small to fill it alone; chunks of a bounded
     length keep members of different lengths balanced.  When the batch
     holds too little fused work (fd3d_batch_set_fused_items) it runs the
     separate sweeps, both with the member's ny_str rows."""
-    if not (fused and case.fspace):
+    if not fused:
         return t
     nyi = case.nyt - 2 - case.nabc
     if nyi <= 0:
@@ -237,7 +237,7 @@ def batch_plan(cases: Iterable[C.Case], fused: bool = True) -> Tuple[int, List[O
     """Common tile height for a batch and each member's shape at that height:
     the height with the least summed sweep time over the measured members,
     among the heights measured for every one of them (16 when none is);
-    full-space members run the fused sweep when fused is set (batch_member)."""
+    members run the fused sweep when fused is set (batch_member)."""
     cases = list(cases)
     tables = [e for e in (lookup(c) for c in cases) if e]
     totals = {}

@@ -56,6 +56,8 @@ struct BatchDomain {
   int nux;            // 64-wide blocks along x over the nucleation patch
   int nout;           // fault-node items of the output stage; station items follow
   int fgx, fgz, nyf;  // fused sweep: tiles along x and z, rows per item
+  int sty, stz0;      // top-plane stresses (free surface with the fused sweep):
+                      // row blocks of tz rows, first plane's index past nabc
 };
 
 struct BatchSeg {
@@ -104,6 +106,13 @@ __device__ __forceinline__ void run_item(const BatchDomain &D, int stage, int b,
     case K_FUVW_W:
       fuvw_w_body<TZ>(P.g, fo, fn, b % D.fpx, b / D.fpx, lx, ly);
       break;
+    case K_STR_TOP: {
+      // one 64 x TZ block of interior (i, j) cells of one of the two top
+      // planes: the one-thread-per-cell body with its rows of BYF lanes
+      const int bx = b % D.fix, r = b / D.fix, jb = r % D.sty, z = r / D.sty;
+      str_interior_body(P.g, P.c, fo, fn, bx, jb * (TZ / BYF), D.stz0 + z, lx, ly);
+      break;
+    }
     case K_STR_INTERIOR:
       str_tiled_body<64, TZ>(P.g, P.c, fo, fn, lds, b % D.gx, (b / D.gx) % D.gz, b / (D.gx * D.gz), lx, ly, D.nys);
       break;
@@ -223,9 +232,10 @@ __global__ void __launch_bounds__(fused::NT) k_batch_fused(BatchArgs A, int seg_
   const int sg = find_seg(A.seg, seg_lo, seg_hi, item);
   const BatchDomain &D = A.dom[A.seg[sg].dom];
   const int b = item - A.seg[sg].start;
-  // members run the fused stage in full space only (describe)
+  // under a free surface the top two planes' stresses are the str_top stage
+  const int kmax_s = D.p.g.nzt - D.p.g.nfs - (D.p.g.fspace ? 0 : 2);
   fused::fused_body(D.p.g, D.p.c, D.p.f[0], D.p.f[1], D.p.ctl, b % D.fgx, (b / D.fgx) % D.fgz, b / (D.fgx * D.fgz),
-                    D.nyf, D.p.g.nzt - D.p.g.nfs, L);
+                    D.nyf, kmax_s, L);
 }
 
 // One stage of every member; the stage is a template argument, so each
@@ -313,10 +323,11 @@ int bfail(fd3d_batch *b, int code, const std::string &m) {
   } while (0)
 
 // Work items of each stage of h at tile height tz.  With fused allowed
-// (staged launches, enough fused work in the batch) a full-space member
-// whose tuning asks for the fused interior sweep (mode bit 2) runs it as
-// its own stage; otherwise the member runs the separate sweeps (work items
-// of one sweep), whatever form it uses alone.
+// (staged launches, enough fused work in the batch) a member whose tuning
+// asks for the fused interior sweep (mode bit 2) runs it as a stage, and
+// under a free surface the top-plane stresses as the str_top stage after
+// the ghost velocities; otherwise the member runs the separate sweeps
+// (work items of one sweep), whatever form it uses alone.
 BatchDomain describe(const fd3d_handle *h, int tz, bool allow_fused) {
   BatchDomain D{};
   D.p = h->p;
@@ -325,7 +336,7 @@ BatchDomain describe(const fd3d_handle *h, int tz, bool allow_fused) {
   const int ni = g.nxt - 2 * g.nabc, nyi = g.nyt - 2 - g.nabc, nk = g.nzt - g.nfs - g.nabc;
   D.gx = cdiv(ni, 64);
   D.gz = cdiv(nk, tz);
-  const bool fz = allow_fused && (h->p.tune.mode & 4) && g.fspace && nyi > 0;
+  const bool fz = allow_fused && (h->p.tune.mode & 4) && nyi > 0;
   D.nyv = std::max(1, h->p.tune.ny_vel);
   D.nys = std::max(1, h->p.tune.ny_str);
   // a member that asks for the fused sweep holds its fused rows in ny_vel;
@@ -356,11 +367,14 @@ BatchDomain describe(const fd3d_handle *h, int tz, bool allow_fused) {
   items[K_ADVANCE] = 1;
   items[K_RATES] = h->p.rates.on ? 1 + cdiv(h->p.rates.NL * h->p.rates.NW, blk) : 0;
   items[K_FUSED] = fz ? D.fgx * D.fgz * cdiv(nyi, D.nyf) : 0;
+  D.sty = cdiv(std::max(nyi, 1), tz);
+  D.stz0 = nk >= 2 ? nk - 2 : 0;
+  items[K_STR_TOP] = (fz && !g.fspace && nk > 0) ? D.fix * D.sty * (nk - D.stz0) : 0;
   for (int s = 0; s < K_COUNT; ++s) { D.n[s] = 0; D.dep[s] = -1; }
   std::vector<KernelId> seq;
   for (KernelId id : h->seq) {
     if (id == K_FUSED && !fz) { seq.push_back(K_VEL_INTERIOR); continue; }
-    if (id == K_STR_TOP) continue;   // free-surface members run the separate sweeps here
+    if (id == K_STR_TOP && !fz) continue;
     seq.push_back(id);
   }
   if (!fz) {

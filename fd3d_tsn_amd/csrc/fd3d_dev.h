@@ -194,9 +194,12 @@ struct Problem {
   Ctl *ctl;
 };
 
+// In the order the stages run within a step (the batch engine launches its
+// stages in this order).
 enum KernelId {
-  K_VEL_INTERIOR = 0, K_VEL_FAULT, K_PML_VEL, K_FUVW_UV, K_FUVW_W,
+  K_VEL_INTERIOR = 0, K_VEL_FAULT, K_PML_VEL,
   K_FUSED,             // interior velocities and stresses in one pass
+  K_FUVW_UV, K_FUVW_W,
   K_STR_TOP,           // with the fused pass and a free surface: stresses of the top two interior planes
   K_STR_INTERIOR, K_STR_FAULT, K_PML_STR, K_FRES,
   K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_RATES, K_NUCLEATION, K_ADVANCE,

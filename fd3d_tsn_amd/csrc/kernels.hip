@@ -16,7 +16,7 @@
 namespace fd3d {
 
 const char *kKernelNames[K_COUNT] = {
-    "vel_interior", "vel_fault", "pml_vel", "fuvw_uv", "fuvw_w", "fused_interior", "str_top",
+    "vel_interior", "vel_fault", "pml_vel", "fused_interior", "fuvw_uv", "fuvw_w", "str_top",
     "str_interior", "str_fault", "pml_str", "fres",
     "traction", "traction_mirror", "tabs", "friction", "outputs", "rates", "nucleation", "advance"};
 

@@ -126,7 +126,7 @@ class Server:
                  reserve_bytes: int = 4 << 30, keep_final: bool = True,
                  log: Optional[Callable[[str], None]] = None, fused: bool = True):
         self.max_slots = max_slots
-        self.fused = fused and mode == capi.STAGED   # full-space members run the fused sweep
+        self.fused = fused and mode == capi.STAGED   # members ask for the fused sweep
         self.chunk = chunk
         self.reserve = reserve_bytes
         self.keep_final = keep_final
@@ -168,8 +168,8 @@ class Server:
                 s = capi.Solver(r.case)
                 if r.bins is not None:
                     s.set_source_bins(*r.bins, nbins=r.steps)
-                # rows per tile for this size class; the fused sweep for
-                # full-space requests
+                # rows per tile for this size class; the fused sweep when
+                # the batch holds enough fused work
                 t = autotune.batch_member(r.case, autotune.choose(r.case, self.tz), self.tz, self.fused)
                 if t is not None:
                     s.set_tuning(*t)

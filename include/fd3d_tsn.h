@@ -244,8 +244,9 @@ int fd3d_batch_step(fd3d_batch *b, int nsteps);
 int fd3d_batch_sync(fd3d_batch *b);
 
 /* Members whose tuning asks for the fused interior sweep (fd3d_set_tuning
- * mode bit 2, full-space problems; ny_vel rows per work item) run it as a
- * stage of the batch when the batch's fused work items reach
+ * mode bit 2; ny_vel rows per work item) run it as a stage of the batch
+ * (under a free surface with the top-plane stresses as a stage of their
+ * own) when the batch's fused work items reach
  * min_items_per_cu per compute unit (default 8; 0: whenever a member asks);
  * otherwise every member runs the separate sweeps, such members with ny_str
  * rows for both.  Results are bit-identical either way.  Marks the step

@@ -46,10 +46,10 @@ def test_batch_plan_minimises_summed_time(cache):
     tz, plan = A.batch_plan([a, b], fused=False)
     assert tz == 8                      # 3.5 + 4.2 + 0.2 + 0.3 < 3 + 4 + 1 + 1
     assert plan[0] == (1, 8, 32, 8, 32) and plan[1] == (1, 8, 32, 8, 32)
-    # with the fused sweep: the full-space member (a) runs it in chunks of at
-    # most 128 rows, the free-surface one (b) keeps the separate sweeps
+    # with the fused sweep: both members (b has a free surface) ask for it,
+    # in chunks of at most 128 rows
     tz, plan = A.batch_plan([a, b])
-    assert plan[0] == (1 | 4, 8, 128, 8, 32) and plan[1] == (1, 8, 32, 8, 32)
+    assert plan[0] == (1 | 4, 8, 128, 8, 32) and plan[1] == (1 | 4, 8, 128, 8, 32)
 
 
 def test_choose_takes_fused_sweep_when_faster(cache):
@@ -78,17 +78,18 @@ def test_fused_ny_minimises_waves_times_rows():
     assert A.fused_ny(c) == 136                                 # 2 chunks: 232 blocks, one wave
 
 
-def test_fused_chosen_with_free_surface_alone_not_in_batches(cache):
+def test_fused_chosen_with_free_surface(cache):
     case = C.c3_fvw()            # free surface
     tab = {"vel": {"16x32": 3.0}, "str": {"16x32": 4.0}, "pml": {}, "fused": {"64": 1.0}}
     json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
     assert A.choose(case)[0] & 4
-    # the batch engine runs free-surface members with the separate sweeps
-    assert A.batch_member(case, A.choose(case, 16), 16) == A.choose(case, 16)
+    # in a batch too, in chunks of at most 128 rows
+    t = A.batch_member(case, A.choose(case, 16), 16)
+    assert t[0] & 4 and t[2] == 128
 
 
 def test_batch_member_fused_chunks():
     small = C.c1_plain_sw()            # full space, 62 interior rows
     assert A.batch_member(small, None, 16) == (1 | 4, 16, 62, 16, 32)
     assert A.batch_member(small, (1, 8, 16, 8, 32), 8, fused=False) == (1, 8, 16, 8, 32)
-    assert A.batch_member(C.c3_fvw(), (1, 8, 16, 8, 32), 8) == (1, 8, 16, 8, 32)   # free surface
+    assert A.batch_member(C.c3_fvw(), (1, 8, 16, 8, 32), 8) == (1 | 4, 8, 128, 8, 32)   # free surface
