@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--batch-mode", default="staged", choices=["persistent", "staged"])
     ap.add_argument("--batch-tz", type=int, default=16)
     ap.add_argument("--batch-fused", default="on", choices=["on", "off"],
-                    help="full-space members of a batch run the fused interior sweep")
+                    help="members of a batch ask for the fused interior sweep")
     ap.add_argument("--batch-n", type=int, default=64, help="domains of the batched workload")
     ap.add_argument("--serve-n", type=int, default=48, help="requests of the served stream sample")
     ap.add_argument("--serve-slots", type=int, default=32)
@@ -176,7 +176,8 @@ def bench_batched(a):
         s.sync()
         plain.append((time.perf_counter() - t1) / a.steps)
     # bytes each member moves per step in the batch (fused members: 72 B per bulk cell)
-    # (the batch engine runs the fused stage for full-space members only)
+    # (the batch engine runs the fused stage when the batch holds at least
+    # 8 fused work items per CU, as the 64-domain workload does)
     step_b = [((MOVED_BULK_FUSED if p is not None and p[0] & 4 else MOVED_BULK) * nb + MOVED_PML * npml)
               for c, p, (_, nb, npml) in zip(cases, plan, (s.step_bytes() for s in solvers))]
     step_plain = [MOVED_BULK * nb + MOVED_PML * npml for _, nb, npml in (s.step_bytes() for s in solvers)]
