@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(64 * TZ) k_batch_persistent(BatchArgs A) {
 }
 
 // The fused interior sweep of every member that runs it: one 64 x 16
-// workgroup per fused tile and y chunk (fused.h).
+// workgroup per fused tile and y chunk (fused.h), tiles in strip order.
 __global__ void __launch_bounds__(fused::NT) k_batch_fused(BatchArgs A, int seg_lo, int seg_hi, int item0, int n) {
   __shared__ fused::FusedLds L;
   const int item = item0 + xcd_remap((int)blockIdx.x, n);
@@ -234,8 +234,10 @@ __global__ void __launch_bounds__(fused::NT) k_batch_fused(BatchArgs A, int seg_
   const int b = item - A.seg[sg].start;
   // under a free surface the top two planes' stresses are the str_top stage
   const int kmax_s = D.p.g.nzt - D.p.g.nfs - (D.p.g.fspace ? 0 : 2);
-  fused::fused_body(D.p.g, D.p.c, D.p.f[0], D.p.f[1], D.p.ctl, b % D.fgx, (b / D.fgx) % D.fgz, b / (D.fgx * D.fgz),
-                    D.nyf, kmax_s, L);
+  const int per = D.fgx * D.fgz, by = b / per;
+  int bx, bz;
+  fused::strip_tile(b - by * per, D.fgx, D.fgz, bx, bz);
+  fused::fused_body(D.p.g, D.p.c, D.p.f[0], D.p.f[1], D.p.ctl, bx, bz, by, D.nyf, kmax_s, L);
 }
 
 // One stage of every member; the stage is a template argument, so each

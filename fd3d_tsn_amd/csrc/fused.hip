@@ -6,24 +6,15 @@ namespace {
 
 using namespace fused;
 
-// Tile order within a y chunk: strips kStrip tiles wide in x, each walked
-// in z, so the 32 blocks an XCD runs at once (xcd_remap gives each XCD a
-// contiguous range) cover 8 z-tiles of 4 x-tiles instead of 2 z-rows of
-// 16 x-tiles.  Measured on the 1024x512x512 sweep (5 processes each): 4.60
-// ms against 4.79 ms for the x-fastest order, though the L2 then fetches
-// slightly more (1.42 against 1.36 times the algorithmic reads); equal on
-// 512x256x256.  Strips 2 tiles wide: 4.53 ms there but 8% slower on
-// 512x256x256; z-fastest order (1 tile wide): 5.93 ms.
-constexpr int kStrip = 4;
-
 __global__ void __launch_bounds__(NT) k_vs_fused(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl, int gx,
                                                        int gz, int gy, int NY) {
   __shared__ FusedLds L;
   const int lid = xcd_remap(blockIdx.x, gx * gz * gy);
   const int kmax_s = g.nzt - g.nfs - (g.fspace ? 0 : 2);
-  const int per = gx * gz, by = lid / per, r = lid - by * per;
-  const int st = r / (kStrip * gz), w = min(kStrip, gx - st * kStrip), q = r - st * kStrip * gz;
-  fused_body(g, c, f0, f1, ctl, st * kStrip + q % w, q / w, by, NY, kmax_s, L);
+  const int per = gx * gz, by = lid / per;
+  int bx, bz;
+  strip_tile(lid - by * per, gx, gz, bx, bz);
+  fused_body(g, c, f0, f1, ctl, bx, bz, by, NY, kmax_s, L);
 }
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
