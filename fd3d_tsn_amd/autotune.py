@@ -190,7 +190,7 @@ def choose(case: C.Case, tz: Optional[int] = None) -> Optional[Tuple[int, int, i
         return None
     mode = 1 | (8 if wv else 0) | (16 if ws else 0)
     if e.get("pml") and e["pml"].get("tiled", 1e30) < e["pml"].get("flat", 1e30):
-        mode = 3
+        mode |= 2   # tiled PML sweeps; the tile-width bits stay
     # the fused sweep (single problems only: the batch engine runs the
     # separate sweeps) when it beats the best separate pair
     # (rows per block from the problem's geometry, fused_ny) when its best

@@ -188,6 +188,7 @@ class Solver:
         if fvw and fault is None:
             fault, fvw_scalars = C.fault_fvw(case)
         self.cfg, self._sta = make_config(case, fvw_scalars, newton_max, ring_steps)
+        self.ring_steps = ring_steps if ring_steps > 0 else 64   # the library's default ring
         self.h = ctypes.c_void_p()
         t1 = clock()
         rc = self.L.fd3d_create(ctypes.byref(self.cfg), ctypes.byref(self.h))

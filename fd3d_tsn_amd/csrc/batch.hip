@@ -283,6 +283,7 @@ __global__ void k_batch_reset(unsigned *cnt, int n) {
 }
 
 }  // namespace
+
 }  // namespace fd3d
 
 struct fd3d_batch {
@@ -523,6 +524,7 @@ hipEvent_t take_event(fd3d_batch *b) {
 
 hipStream_t fd3d_batch_stream_of(const fd3d_batch *b) { return b ? b->stream : nullptr; }
 
+
 extern "C" {
 
 int fd3d_batch_create(int max_problems, int tz, int mode, fd3d_batch **out) {
@@ -598,9 +600,10 @@ int fd3d_batch_sync(fd3d_batch *b) {
   unsigned flag = 0;
   BCHK(b, hipMemcpy(&flag, b->d_cnt + 1, sizeof(unsigned), hipMemcpyDeviceToHost));
   if (flag) return bfail(b, FD3D_E_HIP, "batch: a stage dependency wait ran out of time");
-  for (fd3d_handle *h : b->mem) {
+  for (size_t q = 0; q < b->mem.size(); ++q) {
+    fd3d_handle *h = b->mem[q];
     int rc = fd3d_handle_check_ctl(h, b->stream);
-    if (rc) return bfail(b, rc, h->err);
+    if (rc) return bfail(b, rc, "member " + std::to_string(q) + ": " + h->err);
   }
   return 0;
 }

@@ -540,7 +540,8 @@ static int download3(fd3d_handle *h, float *dst, const float *src) {
 // device keeps one value per plane, so the sweeps read no medium arrays.
 int fd3d_upload_medium(fd3d_handle *h, const float *lam1, const float *mu1, const float *d1) {
   if (!h || !lam1 || !mu1 || !d1) return fail(h, FD3D_E_INVALID, "null argument");
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (h->batch) return fail(h, FD3D_E_STATE, "remove the problem from its batch first");
+  HIPCHK(h, owner_sync(h));
   const int nzt = h->cfg.nzt;
   const size_t plane = (size_t)h->cfg.nxt * h->cfg.nyt;
   std::vector<float> lay(3 * ((size_t)nzt + 2), 0.0f);

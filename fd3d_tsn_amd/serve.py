@@ -59,6 +59,10 @@ class Result:
     cells: int
     final: Dict[str, np.ndarray] = field(repr=False, default_factory=dict)
     bins: Dict[str, np.ndarray] = field(repr=False, default_factory=dict)
+    # per-step slabs and stations of the last steps (Server(keep_outputs=True))
+    outputs: Dict[str, np.ndarray] = field(repr=False, default_factory=dict)
+    # why the request failed (FD3D_E_NEWTON, FD3D_E_NONFINITE, ...), or None
+    error: Optional[str] = None
 
     @property
     def latency(self) -> float:
@@ -110,9 +114,11 @@ def percentile(x, p):
 
 
 def summary(results: List[Result], makespan: float) -> Dict[str, float]:
+    failed = sum(1 for r in results if r.error)
+    results = [r for r in results if not r.error]
     lat = [r.latency for r in results]
     upd = sum(r.cells * r.steps for r in results)
-    return {"requests": len(results), "makespan_s": makespan,
+    return {"requests": len(results), "failed": failed, "makespan_s": makespan,
             "ruptures_per_s": len(results) / makespan if makespan > 0 else float("nan"),
             "Gcell_updates_per_s": upd / makespan / 1e9 if makespan > 0 else float("nan"),
             "latency_p50_s": percentile(lat, 50), "latency_p99_s": percentile(lat, 99),
@@ -124,8 +130,14 @@ class Server:
 
     def __init__(self, max_slots: int = 32, chunk: int = 16, tz: int = 16, mode: int = capi.STAGED,
                  reserve_bytes: int = 4 << 30, keep_final: bool = True,
-                 log: Optional[Callable[[str], None]] = None, fused: bool = True):
+                 log: Optional[Callable[[str], None]] = None, fused: bool = True,
+                 keep_outputs: bool = False, solver_args: Optional[Callable[[Request], Dict]] = None):
         self.max_slots = max_slots
+        # keep_outputs: each result also holds the output slabs and station
+        # samples of its last min(steps, ring) steps (the device ring)
+        self.keep_outputs = keep_outputs
+        # extra capi.Solver arguments per request (e.g. newton_max)
+        self.solver_args = solver_args or (lambda r: {})
         self.fused = fused and mode == capi.STAGED   # members ask for the fused sweep
         self.chunk = chunk
         self.reserve = reserve_bytes
@@ -165,7 +177,7 @@ class Server:
             # admission: a free slot and the footprint in free HBM
             while queue and len(active) + len(ready) < self.max_slots and self._fits(queue[0], 0):
                 r = queue.popleft()
-                s = capi.Solver(r.case)
+                s = capi.Solver(r.case, **self.solver_args(r))
                 if r.bins is not None:
                     s.set_source_bins(*r.bins, nbins=r.steps)
                 # rows per tile for this size class; the fused sweep when
@@ -178,31 +190,57 @@ class Server:
                 raise RuntimeError(f"request {queue[0].rid} does not fit in device memory")
             if n:
                 t1 = clock()
-                self.batch.sync()
+                failed = self._sync()
                 self.gpu_busy_s += clock() - t1
                 self.gpu_step_s += self.batch.elapsed()[0] * 1e-3
                 self.chunks += 1
                 done = []
                 for a in active:
                     a["left"] -= n
-                    if a["left"] == 0:
+                    if a["left"] == 0 or id(a["solver"]) in failed:
                         done.append(a)
                 for a in done:
                     s = a["solver"]
-                    self.batch.remove(s)
-                    fin = s.final() if self.keep_final else {}
                     r = a["req"]
-                    bins = s.source_bins() if r.bins is not None else {}
+                    self.batch.remove(s)
+                    err = failed.get(id(s))
+                    fin = s.final() if self.keep_final and not err else {}
+                    bins = s.source_bins() if r.bins is not None and not err else {}
+                    outs = s.read_outputs(min(r.steps, s.ring_steps)) if self.keep_outputs and not err else {}
                     s.close()
                     results.append(Result(r.rid, r.arrival, a["admitted"], clock() - t0, r.steps, r.case.cells,
-                                          fin, bins))
+                                          fin, bins, outs, err))
                     active.remove(a)
-                    self.log(f"[serve] request {r.rid} done: latency {results[-1].latency:.3f} s, "
-                             f"{len(active)} active, {len(queue)} queued")
+                    what = f"failed ({err})" if err else f"done: latency {results[-1].latency:.3f} s"
+                    self.log(f"[serve] request {r.rid} {what}, {len(active)} active, {len(queue)} queued")
             elif not ready and pending:
                 time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
         self.makespan = clock() - t0
         return results
+
+    def _sync(self) -> Dict[int, str]:
+        """Wait for the chunk.  A member whose rate-and-state Newton solve hit
+        its cap, or whose outputs went non-finite, fails alone: the batch's
+        error names a member, each member's own flags (fd3d_sync) say which,
+        and the others go on (the members do not share state).  Returns
+        {id(solver): error} of the failed members."""
+        try:
+            self.batch.sync()
+            return {}
+        except capi.FD3DError as e:
+            if e.code not in (-6, -7):       # FD3D_E_NEWTON, FD3D_E_NONFINITE
+                raise
+        failed = {}
+        for s in self.batch.members:
+            try:
+                s.sync()
+            except capi.FD3DError as e:
+                if e.code not in (-6, -7):
+                    raise
+                failed[id(s)] = str(e)
+        if not failed:
+            raise RuntimeError("the batch reported a failed member, but no member's flags are set")
+        return failed
 
 
 def run_plain(requests: List[Request], keep_final: bool = True, clock=time.perf_counter,
@@ -233,9 +271,11 @@ def run_plain(requests: List[Request], keep_final: bool = True, clock=time.perf_
 
 
 def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
+    """||a - b|| / ||b||; infinite when either side holds a NaN or Inf."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
-    m = np.isfinite(a) & np.isfinite(b)
-    den = np.sqrt(np.sum(b[m] ** 2))
-    num = np.sqrt(np.sum((a[m] - b[m]) ** 2))
+    if not (np.all(np.isfinite(a)) and np.all(np.isfinite(b))):
+        return float("inf")
+    den = np.sqrt(np.sum(b ** 2))
+    num = np.sqrt(np.sum((a - b) ** 2))
     return float(num / den) if den > 0 else float(num)

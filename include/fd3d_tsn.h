@@ -103,7 +103,8 @@ int fd3d_device_memory(size_t *free_bytes, size_t *total_bytes);
  * The reference fills them plane by plane from the layered crustal model
  * (fd3d_init.f90:240-265); the device keeps one value per z plane, so a
  * medium that varies within a plane is refused with FD3D_E_INVALID.
- * Returns FD3D_E_CFL when max(Vp)*dt/dh > 0.25 (fd3d_init.f90:274-278). */
+ * Returns FD3D_E_CFL when max(Vp)*dt/dh > 0.25 (fd3d_init.f90:274-278);
+ * FD3D_E_STATE while the problem is a batch member. */
 int fd3d_upload_medium(fd3d_handle *h, const float *lam1, const float *mu1, const float *d1);
 
 /* Slip-weakening fault inputs at Z-nodes, nxt*nzt each: striniX, striniZ,
@@ -240,7 +241,10 @@ int fd3d_batch_size(const fd3d_batch *b);
 int fd3d_batch_step(fd3d_batch *b, int nsteps);
 
 /* Wait for queued steps; reports a failed dependency wait (FD3D_E_HIP) and
- * the members' FD3D_E_NEWTON / FD3D_E_NONFINITE flags. */
+ * the members' FD3D_E_NEWTON / FD3D_E_NONFINITE flags, the message naming
+ * the first failing member (its position in the order of fd3d_batch_add).
+ * fd3d_sync on a member reports that member's own flags, so a caller can
+ * remove the failed members and go on stepping the others. */
 int fd3d_batch_sync(fd3d_batch *b);
 
 /* Members whose tuning asks for the fused interior sweep (fd3d_set_tuning

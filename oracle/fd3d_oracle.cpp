@@ -1,12 +1,17 @@
 // TEST INFRASTRUCTURE ONLY -- the oracle (see fd3d_oracle.h).
 //
-// A sequential CPU restatement of one FD3D_TSN time step, written to keep
+// A CPU restatement of one FD3D_TSN time step, written to keep
 // the reference's per-point arithmetic: the same operands, the same
 // left-to-right association, no fused multiply-add (built with
 // -ffp-contract=off), correctly rounded IEEE division and square root.
 // With that, the float mode reproduces the reference's FP32 CPU build bit
 // for bit on the slip-weakening path (checked by tests/test_oracle_vs_ref.py
 // against the flang build in oracle/_ref).
+//
+// The 3D sweeps (dvel, dstres, pml_uvw, pml_xyz) split their outer z loop
+// over OpenMP threads: each cell's update reads only values the sweep does
+// not write and writes only that cell, so the result does not depend on
+// the thread count (OMP_NUM_THREADS=1 gives the sequential program).
 //
 // Routine-by-routine provenance (reference file:line) is given at each
 // function.  The code is organised differently from the reference: the
@@ -254,6 +259,7 @@ struct Oracle {
     const R c1 = R(9) / R(8), c2 = R(-1) / R(24), dth = dt / dh;
     const int ie = nxt - nabc, je = nyt - 2, ke = nzt - nfs;
     const size_t sy = nx, sz = nxy;
+#pragma omp parallel for schedule(static)
     for (int k = nabc + 1; k <= ke; ++k)
       for (int j = nabc + 1; j <= je; ++j)
         for (int i = nabc + 1; i <= ie; ++i) {
@@ -263,6 +269,7 @@ struct Oracle {
                                       c1 * (xy[c] - xy[c - sy]) + c2 * (xy[c + sy] - xy[c - 2 * sy]) +
                                       c1 * (xz[c] - xz[c - sz]) + c2 * (xz[c + sz] - xz[c - 2 * sz]));
         }
+#pragma omp parallel for schedule(static)
     for (int k = nabc + 1; k <= ke; ++k)
       for (int j = nabc + 1; j <= je; ++j)
         for (int i = nabc + 1; i <= ie; ++i) {
@@ -272,6 +279,7 @@ struct Oracle {
                                       c1 * (yy[c + sy] - yy[c]) + c2 * (yy[c + 2 * sy] - yy[c - sy]) +
                                       c1 * (yz[c] - yz[c - sz]) + c2 * (yz[c + sz] - yz[c - 2 * sz]));
         }
+#pragma omp parallel for schedule(static)
     for (int k = nabc + 1; k <= ke; ++k)
       for (int j = nabc + 1; j <= je; ++j)
         for (int i = nabc + 1; i <= ie; ++i) {
@@ -409,6 +417,7 @@ struct Oracle {
   void pml_uvw() {
     const size_t sy = nx, sz = nxy;
     for (auto &r : reg) {
+#pragma omp parallel for schedule(static)
       for (int k = r.kb; k <= r.ke; ++k)
         for (int j = r.jb; j <= r.je_uw; ++j)
           for (int i = r.ib; i <= r.ie; ++i) {
@@ -477,6 +486,7 @@ struct Oracle {
   void dstres() {
     const R c1 = R(9) / R(8), c2 = R(-1) / R(24), dth = dt / dh;
     const size_t sy = nx, sz = nxy;
+#pragma omp parallel for schedule(static)
     for (int k = nabc + 1; k <= nzt - nfs; ++k)
       for (int j = nabc + 1; j <= nyt - 2; ++j)
         for (int i = nabc + 1; i <= nxt - nabc; ++i) {
@@ -593,6 +603,7 @@ struct Oracle {
   void pml_xyz() {
     const size_t sy = nx, sz = nxy;
     for (auto &r : reg) {
+#pragma omp parallel for schedule(static)
       for (int k = r.kb; k <= r.ke; ++k)
         for (int j = r.jb; j <= r.je_uw; ++j)
           for (int i = r.ib; i <= r.ie; ++i) {

@@ -41,6 +41,8 @@ def mix():
         with_stations(C.tiny("fvw", True, seed=43)),
         C.tiny("fvw", False, nabc=4, n=(50, 12, 18), seed=44),
         C.tiny("sw", True, nabc=4, n=(70, 9, 33), seed=45),
+        # 5 fused tiles across: the strip tile order with a narrower last strip
+        C.tiny("sw", True, nabc=4, n=(300, 9, 14), seed=47),
     ]
 
 
@@ -189,8 +191,8 @@ def test_small_fused_batch_runs_separate_sweeps():
 def test_members_join_and_leave():
     """Continuous batching: problems join and leave between chunks; each
     result equals the problem run alone for its own number of steps."""
-    cases = mix()
     plan = [(0, 48), (0, 32), (16, 48), (32, 32), (16, 16)]   # (join step, steps to run)
+    cases = mix()[:len(plan)]
     chunk = 16
     say(f"[batch] join/leave schedule {plan}")
     refs = [alone(c, n) for c, (_, n) in zip(cases, plan)]

@@ -203,6 +203,9 @@ FUSED = [("fs_ny3", dict(nabc=5, n=(70, 17, 21), seed=33), 3, True),
          ("free_ny64", dict(nabc=6, n=(130, 20, 14), seed=38), 64, False),
          ("free_tallz", dict(nabc=4, n=(24, 10, 50), seed=39), 4, False),
          ("free_nz5", dict(nabc=4, n=(40, 12, 5), seed=40), 8, False),
+         # 5 tiles across: a strip of 4 tiles and a narrower last strip
+         ("fs_strips", dict(nabc=4, n=(300, 9, 25), seed=43), 8, True),
+         ("free_strips", dict(nabc=4, n=(300, 9, 25), seed=44), 8, False),
          ("fs_tiledpml", dict(nabc=8, n=(70, 17, 21), seed=41), 8, True, 7),
          ("free_tiledpml", dict(nabc=8, n=(70, 17, 21), seed=42), 8, False, 7)]
 

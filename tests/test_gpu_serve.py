@@ -63,3 +63,31 @@ def test_served_equals_alone():
     say(f"[serve] {stats}")
     assert stats["requests"] == len(reqs)
 
+
+
+def test_failed_member_leaves_the_others_running():
+    """A rate-and-state request whose Newton solve hits its cap (newton_max
+    1) fails alone: it comes back with the error, and every other request
+    of the mix still equals its solo run bit for bit."""
+    reqs = requests()
+    bad = 1
+    assert reqs[bad].case.friction == "fvw"
+    say(f"[serve] request {bad} runs with newton_max 1")
+    plain = {r.rid: r for r in serve.run_plain([r for r in reqs if r.rid != bad])}
+    srv = serve.Server(max_slots=3, chunk=8, tz=8, mode=capi.STAGED, reserve_bytes=0, log=say,
+                       solver_args=lambda r: {"newton_max": 1} if r.rid == bad else {})
+    try:
+        got = {r.rid: r for r in srv.run(reqs)}
+    finally:
+        srv.close()
+    assert sorted(got) == [r.rid for r in reqs]
+    assert got[bad].error is not None and "FD3D_E_NEWTON" in got[bad].error, got[bad].error
+    for rid, ref in plain.items():
+        r = got[rid]
+        assert r.error is None, (rid, r.error)
+        for k, v in ref.final.items():
+            assert np.array_equal(r.final[k], v), f"request {rid} final {k}"
+        for k, v in ref.bins.items():
+            assert np.array_equal(r.bins[k], v), f"request {rid} bins {k}"
+    stats = serve.summary(list(got.values()), srv.makespan)
+    assert stats["failed"] == 1 and stats["requests"] == len(reqs) - 1
