@@ -100,6 +100,7 @@ def load(path: Optional[str] = None):
     L.fd3d_device_memory.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     B = ctypes.c_void_p
     L.fd3d_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(B)]
+    L.fd3d_batch_create_static.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(B)]
     L.fd3d_batch_add.argtypes = [B, H]
     L.fd3d_batch_remove.argtypes = [B, H]
     L.fd3d_batch_size.argtypes = [B]
@@ -358,13 +359,19 @@ class Batch:
     from a captured HIP graph.  Members are Solver objects; their outputs
     are read back with Solver.read_outputs / Solver.final."""
 
-    def __init__(self, max_problems: int, tz: int = 16, mode: int = 1):
+    def __init__(self, max_problems: int, tz: int = 16, mode: int = 1, static: bool = False):
+        """static: the batch keeps its tables in the library's fixed device
+        array and allocates no device memory (fd3d_batch_create_static; staged
+        launches, at most 64 problems, one such batch per GPU at a time)."""
         self.L = load()
         self.b = ctypes.c_void_p()
-        rc = self.L.fd3d_batch_create(max_problems, tz, mode, ctypes.byref(self.b))
+        if static:
+            rc = self.L.fd3d_batch_create_static(max_problems, tz, ctypes.byref(self.b))
+        else:
+            rc = self.L.fd3d_batch_create(max_problems, tz, mode, ctypes.byref(self.b))
         if rc != 0:
             self.b = None
-            raise FD3DError(rc, "fd3d_batch_create failed")
+            raise FD3DError(rc, "fd3d_batch_create_static failed" if static else "fd3d_batch_create failed")
         self.members = []
 
     def _chk(self, rc: int):

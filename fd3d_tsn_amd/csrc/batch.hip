@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -282,7 +283,27 @@ __global__ void k_batch_reset(unsigned *cnt, int n) {
   if (i < n) cnt[kCntBase + i] = 0u;
 }
 
+__global__ void k_zero_u32(unsigned *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 }  // namespace
+
+// Fixed device tables of the batch that allocates nothing
+// (fd3d_batch_create_static): the member descriptors, the segment table and
+// the counters of up to kStaticMaxProblems problems, in one array per GPU
+// that lives until the process ends.  The array's name is a host-side
+// stand-in; host code uses only the device address hipGetSymbolAddress
+// returns for the current device.
+constexpr int kStaticMaxProblems = 64;
+constexpr size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+constexpr size_t kStaticDomBytes = align256(sizeof(BatchDomain) * kStaticMaxProblems);
+constexpr size_t kStaticSegBytes = align256(sizeof(BatchSeg) * K_COUNT * kStaticMaxProblems);
+constexpr size_t kStaticCntBytes = align256(sizeof(unsigned) * (kCntBase + K_COUNT * kStaticMaxProblems));
+constexpr size_t kScratchBytes = kStaticDomBytes + kStaticSegBytes + kStaticCntBytes;
+static_assert(kScratchBytes <= (1u << 20), "the fixed batch tables stay under 1 MiB per GPU");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 }  // namespace fd3d
 
@@ -301,6 +322,7 @@ struct fd3d_batch {
   bool dirty = true;
   bool fvw = false;   // some member uses rate-and-state friction
   int fused_items = 8;   // least fused work items per CU for the fused stage
+  int fixed_dev = -1;    // fd3d_batch_create_static: tables in the device's fixed scratch, else -1
   long long wait_ticks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -472,6 +494,8 @@ int rebuild(fd3d_batch *b) {
   b->dirty = false;
   if (b->mem.empty()) return 0;
   if ((int)b->segs.size() > b->seg_cap) {
+    if (b->fixed_dev >= 0)   // cannot happen: one segment per (member, stage) fits the fixed table
+      return bfail(b, FD3D_E_INVALID, "batch: segment table of the fixed scratch exceeded");
     if (b->d_seg) (void)hipFree(b->d_seg);
     b->d_seg = nullptr;
     b->seg_cap = (int)b->segs.size() * 2;
@@ -524,6 +548,43 @@ hipEvent_t take_event(fd3d_batch *b) {
 
 hipStream_t fd3d_batch_stream_of(const fd3d_batch *b) { return b ? b->stream : nullptr; }
 
+namespace {
+
+// Device address of the fixed scratch, per device, looked up once with
+// that device current; and whether a live batch holds it.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+bool g_scratch_busy[kMaxDevices];
+
+// Claims the current device's scratch; returns its device address or null.
+void *claim_scratch(int &dev, int &code) {
+  code = FD3D_E_HIP;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  if (g_scratch_busy[dev]) {
+    code = FD3D_E_STATE;
+    return nullptr;
+  }
+  g_scratch_busy[dev] = true;
+  code = 0;
+  return g_scratch_ptr[dev];
+}
+
+void release_scratch(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev >= 0 && dev < kMaxDevices) g_scratch_busy[dev] = false;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -548,6 +609,34 @@ int fd3d_batch_create(int max_problems, int tz, int mode, fd3d_batch **out) {
   if (hipMalloc(&b->d_cnt, ncnt * sizeof(unsigned)) != hipSuccess) return bail(FD3D_E_NOMEM);
   if (hipMemsetAsync(b->d_cnt, 0, ncnt * sizeof(unsigned), b->stream) != hipSuccess) return bail(FD3D_E_HIP);
   if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(FD3D_E_HIP);
+  *out = b;
+  return 0;
+}
+
+int fd3d_batch_create_static(int max_problems, int tz, fd3d_batch **out) {
+  if (!out) return FD3D_E_INVALID;
+  *out = nullptr;
+  if (max_problems < 1 || max_problems > kStaticMaxProblems || (tz != 4 && tz != 8 && tz != 16))
+    return FD3D_E_INVALID;
+  int dev = -1, code = 0;
+  void *base = claim_scratch(dev, code);
+  if (!base) return code;
+  fd3d_batch *b = new fd3d_batch();
+  b->maxdom = max_problems;
+  b->tz = tz;
+  b->mode = FD3D_BATCH_STAGED;
+  b->fixed_dev = dev;
+  char *c = static_cast<char *>(base);
+  b->d_dom = reinterpret_cast<BatchDomain *>(c);
+  b->d_seg = reinterpret_cast<BatchSeg *>(c + kStaticDomBytes);
+  b->d_cnt = reinterpret_cast<unsigned *>(c + kStaticDomBytes + kStaticSegBytes);
+  b->seg_cap = K_COUNT * kStaticMaxProblems;
+  auto bail = [&](int rc) { fd3d_batch_destroy(b); return rc; };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(FD3D_E_HIP);
+  // the last batch's counters are still in the array: clear them
+  const int ncnt = kCntBase + K_COUNT * kStaticMaxProblems;
+  hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(ncnt, 256)), dim3(256), 0, b->stream, b->d_cnt, ncnt);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) return bail(FD3D_E_HIP);
   *out = b;
   return 0;
 }
@@ -657,9 +746,13 @@ void fd3d_batch_destroy(fd3d_batch *b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (auto &e : b->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto e : b->pool) (void)hipEventDestroy(e);
-  if (b->d_dom) (void)hipFree(b->d_dom);
-  if (b->d_seg) (void)hipFree(b->d_seg);
-  if (b->d_cnt) (void)hipFree(b->d_cnt);
+  if (b->fixed_dev >= 0) {
+    release_scratch(b->fixed_dev);   // the tables live in the fixed scratch
+  } else {
+    if (b->d_dom) (void)hipFree(b->d_dom);
+    if (b->d_seg) (void)hipFree(b->d_seg);
+    if (b->d_cnt) (void)hipFree(b->d_cnt);
+  }
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }

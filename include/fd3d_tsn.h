@@ -230,6 +230,21 @@ enum fd3d_batch_mode {
  * mode FD3D_BATCH_PERSISTENT is refused with FD3D_E_STATE for now. */
 int fd3d_batch_create(int max_problems, int tz, int mode, fd3d_batch **out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (a serving loop that must not call hipMalloc between requests,
+ * code under HIP graph capture, harnesses that forbid allocation): the same
+ * batch as fd3d_batch_create(max_problems, tz, FD3D_BATCH_STAGED, out),
+ * whose member descriptors, segment table and counters live in a fixed
+ * device array of the library (one per GPU, 125 KiB, held until the
+ * process ends) instead of device allocations.  Creating, adding, removing,
+ * stepping and destroying such a batch call no hipMalloc or hipFree; the
+ * members' own memory comes from fd3d_create as before.  At most 64
+ * problems (max_problems > 64: FD3D_E_INVALID).  One such batch per GPU at
+ * a time: the array is shared by every caller on that GPU, so a second one
+ * while the first lives gets FD3D_E_STATE (overlapping use would race).
+ * Results are bit-identical to the allocating batch. */
+int fd3d_batch_create_static(int max_problems, int tz, fd3d_batch **out);
+
 /* Add an initialised problem (fd3d_init_state done).  While it is a member,
  * fd3d_step, fd3d_set_tuning and fd3d_init_state on it fail with
  * FD3D_E_STATE; downloads and fd3d_read_outputs wait for the batch. */
