@@ -79,6 +79,8 @@ def load(path: Optional[str] = None):
     L.fd3d_init_state.argtypes = [H]
     L.fd3d_step.argtypes = [H, ctypes.c_int, _FP, _FP]
     L.fd3d_sync.argtypes = [H]
+    L.fd3d_step_begin.argtypes = [H, ctypes.c_int]
+    L.fd3d_step_wait.argtypes = [H, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_FP), ctypes.POINTER(_FP)]
     L.fd3d_steps_done.argtypes = [H]
     L.fd3d_download_final.argtypes = [H, _FP]
     L.fd3d_download_field.argtypes = [H, ctypes.c_int, _FP]
@@ -263,6 +265,44 @@ class Solver:
             res[name] = out[:, o].transpose(0, 2, 1)
         res["seis"] = seis[:, :nsta, :]
         return res
+
+    def step_begin(self, nsteps: int):
+        """Advance nsteps (<= ring_steps/2) and queue the copy of their
+        outputs to pinned host memory; returns at once (fd3d_step_begin)."""
+        self._chk(self.L.fd3d_step_begin(self.h, nsteps))
+
+    def step_wait(self, copy: bool = True) -> Dict[str, np.ndarray]:
+        """Outputs of the oldest chunk in flight (fd3d_step_wait), shaped as
+        step() returns them.  copy=False returns views of the handle's pinned
+        buffer, valid until the step_begin after the next."""
+        n = ctypes.c_int()
+        po, ps = _FP(), _FP()
+        self._chk(self.L.fd3d_step_wait(self.h, ctypes.byref(n), ctypes.byref(po), ctypes.byref(ps)))
+        case = self.case
+        nxT, nzT = case.fault_shape
+        nsta = len(case.stations)
+        out = np.ctypeslib.as_array(po, shape=(n.value, NOUT, nzT, nxT))
+        seis = np.ctypeslib.as_array(ps, shape=(n.value, max(nsta, 1), 3))
+        if copy:
+            out, seis = out.copy(), seis.copy()
+        res = {name: out[:, o].transpose(0, 2, 1) for o, name in enumerate(OUT_NAMES)}
+        res["seis"] = seis[:, :nsta, :]
+        return res
+
+    def stream(self, nsteps: int, chunk: int = 32):
+        """Advance nsteps, yielding each chunk's outputs while the GPU
+        computes the next chunk (the overlapped output stream)."""
+        chunk = max(1, min(chunk, self.ring_steps // 2))
+        left = nsteps
+        inflight = 0
+        while left > 0 or inflight:
+            while left > 0 and inflight < 2:
+                n = min(chunk, left)
+                self.step_begin(n)
+                left -= n
+                inflight += 1
+            yield self.step_wait()
+            inflight -= 1
 
     def read_outputs(self, nsteps: int) -> Dict[str, np.ndarray]:
         """Slabs and stations of the last nsteps steps from the device ring

@@ -645,6 +645,7 @@ int fd3d_batch_add(fd3d_batch *b, fd3d_handle *h) {
   if (!b || !h) return FD3D_E_INVALID;
   if (!h->inited) return bfail(b, FD3D_E_STATE, "fd3d_init_state the problem first");
   if (h->batch) return bfail(b, FD3D_E_STATE, "the problem already belongs to a batch");
+  if (h->begun != h->waited) return bfail(b, FD3D_E_STATE, "the problem has output chunks in flight");
   if ((int)b->mem.size() >= b->maxdom) return bfail(b, FD3D_E_INVALID, "batch is full");
   BCHK(b, hipStreamSynchronize(h->stream));
   BCHK(b, hipStreamSynchronize(b->stream));

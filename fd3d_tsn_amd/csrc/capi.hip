@@ -609,6 +609,7 @@ int fd3d_init_state(fd3d_handle *h) {
   if (!h) return FD3D_E_INVALID;
   if (!h->medium_ok || !h->fault_ok) return fail(h, FD3D_E_STATE, "upload the medium and the fault first");
   if (h->batch) return fail(h, FD3D_E_STATE, "remove the problem from its batch first");
+  if (h->begun != h->waited) return fail(h, FD3D_E_STATE, "output chunks in flight: fd3d_step_wait first");
   Problem &p = h->p;
   const Dims &g = p.g;
   const size_t n3 = (size_t)g.n3;
@@ -754,6 +755,7 @@ int fd3d_step(fd3d_handle *h, int nsteps, float *out, float *seis) {
   if (!h->inited) return fail(h, FD3D_E_STATE, "fd3d_init_state first");
   if (h->batch) return fail(h, FD3D_E_STATE, "the problem is advanced by a batch (fd3d_batch_step)");
   if (nsteps < 0) return fail(h, FD3D_E_INVALID, "negative step count");
+  if (h->begun != h->waited) return fail(h, FD3D_E_STATE, "output chunks in flight: fd3d_step_wait first");
   const int ring = h->cfg.ring_steps;
   const size_t slab = (size_t)h->p.g.nxT * h->p.g.nzT;
   const int nsta = h->cfg.nstations;
@@ -796,6 +798,98 @@ int fd3d_sync(fd3d_handle *h) {
   if (!h) return FD3D_E_INVALID;
   HIPCHK(h, owner_sync(h));
   return check_ctl(h);
+}
+
+// Pinned buffers, events and copy stream of the output stream, made at the
+// first fd3d_step_begin.
+static int stream_setup(fd3d_handle *h) {
+  if (h->copy) return 0;
+  const size_t slab = (size_t)h->p.g.nxT * h->p.g.nzT;
+  const size_t half = (size_t)h->cfg.ring_steps / 2;
+  const size_t floats = half * FD3D_NOUT * slab + half * (size_t)std::max(1, h->cfg.nstations) * 3;
+  for (auto &c : h->chunk) {
+    void *p = nullptr;
+    HIPCHK(h, hipHostMalloc(&p, floats * sizeof(float) + sizeof(Ctl), hipHostMallocDefault));
+    c.host = static_cast<float *>(p);
+    c.ctl = reinterpret_cast<Ctl *>(c.host + floats);
+    HIPCHK(h, hipEventCreateWithFlags(&c.computed, hipEventDisableTiming));
+    HIPCHK(h, hipEventCreateWithFlags(&c.copied, hipEventDisableTiming));
+  }
+  HIPCHK(h, hipStreamCreateWithFlags(&h->copy, hipStreamNonBlocking));
+  return 0;
+}
+
+// Advance nsteps and queue the copy of their outputs (see the header).
+int fd3d_step_begin(fd3d_handle *h, int nsteps) {
+  if (!h) return FD3D_E_INVALID;
+  if (!h->inited) return fail(h, FD3D_E_STATE, "fd3d_init_state first");
+  if (h->batch) return fail(h, FD3D_E_STATE, "the problem is advanced by a batch (fd3d_batch_step)");
+  const int half = h->cfg.ring_steps / 2;
+  if (nsteps < 1 || nsteps > half)
+    return fail(h, FD3D_E_INVALID, "fd3d_step_begin: 1 <= nsteps <= ring_steps/2 = " + std::to_string(half));
+  if (h->begun - h->waited >= 2)
+    return fail(h, FD3D_E_STATE, "two chunks are in flight: fd3d_step_wait first");
+  int rc = stream_setup(h);
+  if (rc) return rc;
+  fd3d_handle::Chunk &c = h->chunk[h->begun & 1];
+  // the chunk that used this host buffer (two chunks ago) has been handed
+  // out, and its copy has finished; the device ring slots of this chunk
+  // were last copied by that chunk or an earlier one (two chunks span at
+  // most ring_steps slots), so the steps may overwrite them
+  if (h->begun >= 2) HIPCHK(h, hipStreamWaitEvent(h->stream, c.copied, 0));
+  const int ring = h->cfg.ring_steps;
+  c.first = h->steps + 1;
+  c.n = nsteps;
+  if (h->prof) {
+    for (int s = 0; s < nsteps; ++s)
+      for (KernelId id : h->seq)
+        if ((rc = launch_one(h, id))) return rc;
+  } else {
+    if (!h->gexec && (rc = capture_step(h))) return rc;
+    for (int s = 0; s < nsteps; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+  }
+  HIPCHK(h, hipEventRecord(c.computed, h->stream));
+  HIPCHK(h, hipStreamWaitEvent(h->copy, c.computed, 0));
+  const size_t slab = (size_t)h->p.g.nxT * h->p.g.nzT;
+  const int nsta = h->cfg.nstations;
+  float *seis = c.host + (size_t)half * FD3D_NOUT * slab;
+  // step s sits in ring slot (s-1) % ring: at most two contiguous runs
+  int done = 0;
+  while (done < nsteps) {
+    const int slot = (c.first - 1 + done) % ring;
+    const int n = std::min(nsteps - done, ring - slot);
+    HIPCHK(h, hipMemcpyAsync(c.host + (size_t)done * FD3D_NOUT * slab, h->p.out.ring + (size_t)slot * FD3D_NOUT * slab,
+                             (size_t)n * FD3D_NOUT * slab * sizeof(float), hipMemcpyDeviceToHost, h->copy));
+    if (nsta > 0)
+      HIPCHK(h, hipMemcpyAsync(seis + (size_t)done * nsta * 3, h->p.out.seis + (size_t)slot * nsta * 3,
+                               (size_t)n * nsta * 3 * sizeof(float), hipMemcpyDeviceToHost, h->copy));
+    done += n;
+  }
+  // the flags as of this chunk (or a later step: the next chunk may already
+  // run; they only grow)
+  HIPCHK(h, hipMemcpyAsync(c.ctl, h->p.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->copy));
+  HIPCHK(h, hipEventRecord(c.copied, h->copy));
+  h->steps += nsteps;
+  h->begun += 1;
+  return 0;
+}
+
+// Wait for the oldest chunk's copy and hand out its host buffer.
+int fd3d_step_wait(fd3d_handle *h, int *nsteps, const float **out, const float **seis) {
+  if (!h) return FD3D_E_INVALID;
+  if (h->waited >= h->begun) return fail(h, FD3D_E_STATE, "no chunk in flight: fd3d_step_begin first");
+  fd3d_handle::Chunk &c = h->chunk[h->waited & 1];
+  HIPCHK(h, hipEventSynchronize(c.copied));
+  h->waited += 1;
+  const size_t slab = (size_t)h->p.g.nxT * h->p.g.nzT;
+  if (nsteps) *nsteps = c.n;
+  if (out) *out = c.host;
+  if (seis) *seis = c.host + (size_t)(h->cfg.ring_steps / 2) * FD3D_NOUT * slab;
+  if (c.ctl->newton_capped > 0)
+    return fail(h, FD3D_E_NEWTON, "rate-and-state Newton solve hit the iteration cap at " +
+                                      std::to_string(c.ctl->newton_capped) + " node-steps");
+  if (c.ctl->nonfinite) return fail(h, FD3D_E_NONFINITE, "non-finite slip rate in the fault outputs");
+  return 0;
 }
 
 int fd3d_steps_done(const fd3d_handle *h) { return h ? h->steps : 0; }
@@ -983,6 +1077,13 @@ void fd3d_destroy(fd3d_handle *h) {
   for (auto &e : h->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto e : h->pool) (void)hipEventDestroy(e);
   drop_graph(h);
+  if (h->copy) (void)hipStreamSynchronize(h->copy);
+  for (auto &c : h->chunk) {
+    if (c.host) (void)hipHostFree(c.host);
+    if (c.computed) (void)hipEventDestroy(c.computed);
+    if (c.copied) (void)hipEventDestroy(c.copied);
+  }
+  if (h->copy) (void)hipStreamDestroy(h->copy);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->fork) (void)hipEventDestroy(h->fork);
   if (h->join) (void)hipEventDestroy(h->join);

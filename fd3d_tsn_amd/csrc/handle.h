@@ -44,6 +44,18 @@ struct fd3d_handle {
   hipEvent_t fork = nullptr, join = nullptr;
   // the batch this problem is advanced in (fd3d_batch_add), or null
   fd3d_batch *batch = nullptr;
+  // overlapped output stream (fd3d_step_begin / fd3d_step_wait): two pinned
+  // host buffers, each holding one chunk's slabs, station samples and the
+  // control block after the chunk; the copies run on their own stream
+  struct Chunk {
+    float *host = nullptr;       // [ring/2][FD3D_NOUT][slab] slabs, then [ring/2][nsta][3] samples
+    Ctl *ctl = nullptr;          // control block after the chunk
+    hipEvent_t computed = nullptr, copied = nullptr;
+    int first = 0, n = 0;        // first step (1-based) and step count of the chunk
+  };
+  Chunk chunk[2];
+  hipStream_t copy = nullptr;
+  long long begun = 0, waited = 0;   // chunks begun and handed out, in order
 };
 
 

@@ -49,8 +49,10 @@ program fd3d_tsn_driver
   type(fd3d_config) :: cfg
   type(c_ptr) :: h
   integer(c_int) :: rc
-  integer :: i, k, n, it, chunk, nxs, nzs, ring, nout
-  real(c_float), allocatable, target :: outbuf(:), seisbuf(:)
+  integer :: i, k, n, it, chunk, nxs, nzs, ring, nout, nbegun, inflight
+  integer(c_int) :: nw
+  type(c_ptr) :: pout, pseis
+  real(c_float), pointer :: outbuf(:), seisbuf(:)
   real(c_float), allocatable :: final7(:)
   real, allocatable :: T0X(:, :), T0Z(:, :)
   real :: maxX, maxZ, time
@@ -125,28 +127,40 @@ program fd3d_tsn_driver
   call interp_t0()
   call write_inputs_echo()
 
-  ! ---- time loop (fd3d_theo.f90:187-602), outputs streamed chunk by chunk
+  ! ---- time loop (fd3d_theo.f90:187-602).  Outputs stream out chunk by
+  ! chunk: while the files of one chunk are written, the GPU computes the
+  ! next (fd3d_step_begin / fd3d_step_wait, pinned buffers of the library)
   nxs = nxt - 2 * nabc
   nzs = nzt - nfs - nabc
   nout = merge(5, 4, is_fvw)
   ring = 64
-  allocate (outbuf(int(ring, 8) * FD3D_NOUT * nxs * nzs), seisbuf(max(1, ring * nstations * 3)))
+  chunk = ring / 2
   call open_outputs()
   call cpu_time(cput1)
   call system_clock(c1, crate)
   it = 0
-  do while (it < ntfd)
-    chunk = min(ring, ntfd - it)
-    if (nstations > 0) then
-      rc = fd3d_step(h, chunk, c_loc(outbuf), c_loc(seisbuf))
-    else
-      rc = fd3d_step(h, chunk, c_loc(outbuf), c_null_ptr)
-    end if
-    call fd3d_check(rc, h, 'fd3d_step')
-    do n = 1, chunk
+  nbegun = 0
+  inflight = 0
+  do while (nbegun < ntfd .and. inflight < 2)
+    call fd3d_check(fd3d_step_begin(h, min(chunk, ntfd - nbegun)), h, 'fd3d_step_begin')
+    nbegun = nbegun + min(chunk, ntfd - nbegun)
+    inflight = inflight + 1
+  end do
+  do while (inflight > 0)
+    call fd3d_check(fd3d_step_wait(h, nw, pout, pseis), h, 'fd3d_step_wait')
+    inflight = inflight - 1
+    call c_f_pointer(pout, outbuf, [int(nw, 8) * FD3D_NOUT * nxs * nzs])
+    call c_f_pointer(pseis, seisbuf, [max(1, nw * nstations * 3)])
+    do n = 1, nw
       call write_step(n, it + n)
     end do
-    it = it + chunk
+    it = it + nw
+    ! the buffer just written is the one the chunk after next reuses
+    if (nbegun < ntfd) then
+      call fd3d_check(fd3d_step_begin(h, min(chunk, ntfd - nbegun)), h, 'fd3d_step_begin')
+      nbegun = nbegun + min(chunk, ntfd - nbegun)
+      inflight = inflight + 1
+    end if
   end do
   call system_clock(c2)
   call cpu_time(cput2)

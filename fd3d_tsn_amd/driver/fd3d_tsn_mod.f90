@@ -52,6 +52,17 @@ module fd3d_tsn_mod
       integer(c_int), value :: nsteps
       type(c_ptr), value :: out, seis
     end function
+    integer(c_int) function fd3d_step_begin(h, nsteps) bind(C, name='fd3d_step_begin')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+    end function
+    integer(c_int) function fd3d_step_wait(h, nsteps, out, seis) bind(C, name='fd3d_step_wait')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(out) :: nsteps
+      type(c_ptr), intent(out) :: out, seis
+    end function
     integer(c_int) function fd3d_download_final(h, out7) bind(C, name='fd3d_download_final')
       import :: c_int, c_ptr, c_float
       type(c_ptr), value :: h

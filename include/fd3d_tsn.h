@@ -138,6 +138,30 @@ int fd3d_step(fd3d_handle *h, int nsteps, float *out, float *seis);
 /* Wait for queued steps; reports FD3D_E_NEWTON / FD3D_E_NONFINITE. */
 int fd3d_sync(fd3d_handle *h);
 
+/* Overlapped output stream.  The reference copies the fault slabs to the
+ * host and writes the result/ .res files every step (fd3d_theo.f90:199-204,545-558),
+ * with the GPU idle meanwhile.  Here the caller writes one chunk of steps
+ * while the GPU computes the next:
+ *   fd3d_step_begin(h, n)   advance n steps (1 <= n <= ring_steps/2) and
+ *                           queue the copy of their output slabs, station
+ *                           samples and error flags into one of two pinned
+ *                           host buffers owned by the handle, on a copy
+ *                           stream of its own; returns without waiting.
+ *                           At most two chunks are in flight (FD3D_E_STATE).
+ *   fd3d_step_wait(h, &n, &out, &seis)
+ *                           wait for the oldest chunk's copy; *out points to
+ *                           n*FD3D_NOUT*nxtT*nztT floats in the layout of
+ *                           fd3d_step's out, *seis to n*nstations*3 floats.
+ *                           Valid until the fd3d_step_begin after the next
+ *                           (the buffer is then reused).  Returns
+ *                           FD3D_E_NEWTON / FD3D_E_NONFINITE from the flags
+ *                           after that chunk.
+ * Usage: begin(c0); begin(c1); loop { wait -> write; begin(next) }; wait the
+ * rest.  fd3d_step, fd3d_init_state and fd3d_batch_add refuse a handle with
+ * chunks in flight. */
+int fd3d_step_begin(fd3d_handle *h, int nsteps);
+int fd3d_step_wait(fd3d_handle *h, int *nsteps, const float **out, const float **seis);
+
 /* Steps taken since fd3d_init_state. */
 int fd3d_steps_done(const fd3d_handle *h);
 

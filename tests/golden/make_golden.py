@@ -11,7 +11,8 @@ on small seeded problems and stores what the reference wrote:
 * rate-and-state (FP64 build): the same final arrays and stations, plus the
   time histories of every output at a sub-sampled set of fault nodes.
 
-Usage:  python tests/golden/make_golden.py   (needs oracle/_ref built)
+Usage:  python tests/golden/make_golden.py          (needs oracle/_ref built)
+        python tests/golden/make_golden.py tpv104   (the rate-and-state example only)
 """
 import dataclasses
 import hashlib
@@ -128,6 +129,54 @@ def example_fixtures(manifest):
         shutil.rmtree(wd)
 
 
+# Rate-and-state example: the reference's -fdefault-real-8 build (the -r8 of
+# examples/TPV104/compile.sh:13) on examples/TPV104 with the time loop cut
+# to 100 steps.  Its .res streams hold 8-byte reals; the fixture keeps each
+# per-step stream (psi.res included, fd3d_theo.f90:546-553) at every 10th
+# fault node in x and z, and the final slip files (text) likewise, as
+# float32 (rounding 6e-8, far below the 1e-4 bar the driver is held to).
+FVW_EXAMPLE = ("files_tpv104_100", "TPV104", "fd3d_ref_fvw_free", "tpv104", 100, 10)
+FVW_STREAMS = ("sliprateX", "sliprateZ", "shearstressX", "shearstressZ", "psi")
+
+
+def read_text_column(path):
+    return np.asarray([float(x) for x in open(path).read().split()], dtype=np.float64)
+
+
+def fvw_example_fixture(manifest):
+    import shutil
+    import subprocess
+    import tempfile
+    name, example, variant, setup, steps, every = FVW_EXAMPLE
+    print(f"[golden] {name}: {example} ({setup}, {variant}), {steps} steps", flush=True)
+    wd = tempfile.mkdtemp(prefix="fd3dgold_")
+    src = os.path.join(HERE, "examples", example)
+    for f in os.listdir(src):
+        shutil.copy(os.path.join(src, f), os.path.join(wd, f))
+    lines = open(os.path.join(wd, "inputfd3d.dat")).read().split("\n")
+    lines[2] = f"{steps}"
+    open(os.path.join(wd, "inputfd3d.dat"), "w").write("\n".join(lines))
+    open(os.path.join(wd, "harness_setup.txt"), "w").write(setup + "\n")
+    os.makedirs(os.path.join(wd, "result"))
+    p = subprocess.run([os.path.join(refrun.REF_DIR, variant)], cwd=wd, capture_output=True, text=True,
+                       preexec_fn=refrun._big_stack)
+    assert p.returncode == 0, p.stdout[-2000:]
+    nxT, nyT, nzT = (int(v) for v in lines[0].split()[:3])
+    data = {}
+    for n in FVW_STREAMS:
+        a = np.fromfile(os.path.join(wd, "result", f"{n}.res"), dtype=np.float64).reshape(steps, nzT, nxT)
+        data[f"hist_{n}"] = a[:, ::every, ::every].astype(np.float32)
+    for n in ("slipX", "slipZ"):
+        a = read_text_column(os.path.join(wd, "result", f"{n}.res"))
+        data[f"text_{n}"] = a.astype(np.float32)
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **data)
+    shutil.rmtree(wd)
+    manifest[name] = {"example": example, "setup": setup, "steps": steps, "reference_build": variant,
+                      "precision": "float64 (-fdefault-real-8)", "node_stride": every, "file": os.path.basename(path),
+                      "sha256": hashlib.sha256(open(path, "rb").read()).hexdigest()}
+
+
 def rates_cases():
     """Inversion accumulators (MSRX, MSRZ, MomentRate) of the reference's time
     loop, with a 4 x 2 patch grid and 0.04 s bins set through the harness."""
@@ -151,7 +200,16 @@ def rates_fixtures(manifest):
 
 
 def main():
+    if sys.argv[1:] == ["tpv104"]:
+        # only the rate-and-state example fixture, into the existing manifest
+        path = os.path.join(HERE, "manifest.json")
+        manifest = json.load(open(path))
+        fvw_example_fixture(manifest)
+        with open(path, "w") as f:
+            json.dump(manifest, f, indent=1, sort_keys=True)
+        return
     manifest = {}
+    fvw_example_fixture(manifest)
     rates_fixtures(manifest)
     example_fixtures(manifest)
     driver_fixtures(manifest)
