@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="large_sw", choices=["large_sw", "plain_sw", "fvw", "batched", "served"])
+    ap.add_argument("--workload", default="large_sw", choices=["large_sw", "plain_sw", "fvw", "batched", "served", "served-cpu"])
     ap.add_argument("--batch-mode", default="staged", choices=["persistent", "staged"])
     ap.add_argument("--batch-tz", type=int, default=16)
     ap.add_argument("--batch-fused", default="on", choices=["on", "off"],
@@ -60,6 +60,12 @@ def parse():
     ap.add_argument("--serve-n", type=int, default=48, help="requests of the served stream sample")
     ap.add_argument("--serve-slots", type=int, default=32)
     ap.add_argument("--serve-chunk", type=int, default=16)
+    ap.add_argument("--serve-which", default="both", choices=["both", "served", "plain"],
+                    help="served workload: the serving loop, the back-to-back plain solves, or both")
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="served-cpu workload: concurrent reference processes (at most the host's cores)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 64-domain batched leg of the default line")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10,
                     help="time steps of the reference CPU build on the same grid (about 2 s each on "
@@ -135,8 +141,11 @@ def cpu_baseline(case, steps):
                 "sample": f"{case.name}: {case.nxt}x{case.nyt}x{case.nzt} cells, {steps} time steps, "
                           f"reference Fortran built with flang -O3 -ffast-math, serial, pinned to core 0; "
                           f"time = its own 'CPU TIME OF TIME LOOP' ({t:.2f} s)"}
-    import numpy as np
+    import ctypes
     import oracle_lib as O
+    O.lib()
+    # the oracle's sweeps use OpenMP threads; the baseline is one core
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(1)
     o = O.Oracle(sample, O.F32)
     t0 = time.perf_counter()
     o.step(steps, want_out=False)
@@ -145,7 +154,7 @@ def cpu_baseline(case, steps):
             "sample": f"{case.name}: {steps} steps of the C++ oracle restatement, serial ({t:.2f} s)"}
 
 
-def bench_batched(a):
+def bench_batched(a, emit=True):
     """Batched uneven domains (64 slip-weakening ruptures, sides 96..320,
     1500 steps each): time per step of the batch against the sum of the
     per-step times of each domain stepped alone (back-to-back plain
@@ -227,7 +236,9 @@ def bench_batched(a):
                                   "setup_s_per_rupture": sum(setup_s) / len(setup_s),
                                   "setup_phases_s": phases},
            "speedup_vs_plain": t_plain / wall}
-    print(json.dumps(out))
+    if emit:
+        print(json.dumps(out))
+    return out
 
 
 def bench_served(a):
@@ -245,32 +256,132 @@ def bench_served(a):
     with capi.Solver(reqs[0].case.replace(nxtT=32, nytT=16, nztT=16, ntfd=4)) as s:
         s.step(4, outputs=False)
         s.sync()
-    t1 = time.perf_counter()
-    plain = serve.run_plain(reqs)
-    print(f"[bench] plain back-to-back {time.perf_counter()-t1:.1f} s", file=sys.stderr, flush=True)
-    srv = serve.Server(max_slots=a.serve_slots, chunk=a.serve_chunk, tz=a.batch_tz, fused=a.batch_fused == "on")
-    t1 = time.perf_counter()
-    got = srv.run(reqs)
-    print(f"[bench] served {time.perf_counter()-t1:.1f} s", file=sys.stderr, flush=True)
-    srv.close()
-    ps = serve.summary(plain, max(r.finished for r in plain))
-    ss = serve.summary(got, srv.makespan)
-    pmap = {r.rid: r for r in plain}
-    l2 = 0.0
-    for r in got:
-        for k, v in pmap[r.rid].final.items():
-            l2 = max(l2, serve.rel_l2(r.final[k], v))
-    out = {"metric": "served ruptures/s (Poisson stream, continuous batching)", "value": ss["ruptures_per_s"],
+    def progress(tag, every=25):
+        n = [0]
+
+        def log(msg):
+            if "done" in msg or "failed" in msg or tag == "plain":
+                n[0] += 1
+                if n[0] % every == 0 or n[0] == len(reqs):
+                    print(f"[bench] {tag}: {n[0]} of {len(reqs)} requests finished", file=sys.stderr, flush=True)
+        return log
+
+    plain, got, srv = None, None, None
+    keep = a.serve_n <= 200     # final arrays of both legs for the per-request comparison
+    if a.serve_which in ("both", "plain"):
+        t1 = time.perf_counter()
+        plain = serve.run_plain(reqs, keep_final=keep, log=progress("plain"))
+        print(f"[bench] plain back-to-back {time.perf_counter()-t1:.1f} s", file=sys.stderr, flush=True)
+    if a.serve_which in ("both", "served"):
+        srv = serve.Server(max_slots=a.serve_slots, chunk=a.serve_chunk, tz=a.batch_tz, fused=a.batch_fused == "on",
+                           keep_final=keep, log=progress("served"))
+        t1 = time.perf_counter()
+        got = srv.run(reqs)
+        print(f"[bench] served {time.perf_counter()-t1:.1f} s", file=sys.stderr, flush=True)
+        srv.close()
+    ps = serve.summary(plain, max(r.finished for r in plain)) if plain else None
+    ss = serve.summary(got, srv.makespan) if got else None
+    l2 = None
+    if plain and got and keep:
+        pmap = {r.rid: r for r in plain}
+        l2 = 0.0
+        for r in got:
+            for k, v in pmap[r.rid].final.items():
+                l2 = max(l2, serve.rel_l2(r.final[k], v))
+    lead = ss or ps
+    out = {"metric": "served ruptures/s (Poisson stream, continuous batching)" if ss else
+                     "back-to-back plain ruptures/s (Poisson stream)", "value": lead["ruptures_per_s"],
            "unit": "ruptures/s", "n_gpus": 1, "steps": None, "warmup": 0,
            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 (rate-and-state Newton and state in f64)", "data": "synthetic",
            "config": {"workload": f"served_stream_first_{len(reqs)}_of_2000", "requests": len(reqs),
                       "rate_per_s": 20.0, "slots": a.serve_slots, "chunk_steps": a.serve_chunk,
                       "cell_updates": upd},
-           "served": dict(ss, gpu_step_s=srv.gpu_step_s, host_wait_s=srv.gpu_busy_s, chunks=srv.chunks),
+           "served": dict(ss, gpu_step_s=srv.gpu_step_s, host_wait_s=srv.gpu_busy_s, chunks=srv.chunks) if ss else None,
            "plain_back_to_back": ps,
-           "speedup_vs_plain": ss["ruptures_per_s"] / ps["ruptures_per_s"],
+           "speedup_vs_plain": ss["ruptures_per_s"] / ps["ruptures_per_s"] if ss and ps else None,
            "max_rel_l2_vs_plain": l2}
+    print(json.dumps(out))
+
+
+def bench_served_cpu(a):
+    """The reference CPU build on the served mix: the first --serve-n
+    requests of the Poisson stream (full grids and step counts), run by
+    --cpu-procs concurrent reference processes, each pinned to its own host
+    core (the reference is serial); aggregate ruptures/s and cell-updates/s.
+    The mix's rate follows from the cell-update throughput and the mean work
+    per request of the whole 2000-request stream."""
+    import queue
+    import threading
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refrun
+    from fd3d_tsn_amd import serve
+    allreq = serve.c5_requests(2000)
+    reqs = allreq[:a.serve_n]
+    ncpu = os.cpu_count() or 1
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    nproc = max(1, min(a.cpu_procs, ncpu, len(reqs)))
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    cores = sorted(os.sched_getaffinity(0))[:nproc] if hasattr(os, "sched_getaffinity") else list(range(nproc))
+    todo = queue.Queue()
+    for r in reqs:
+        todo.put(r)
+    done, errors = [], []
+    lock = threading.Lock()
+    t0 = time.perf_counter()
+
+    def worker(core):
+        while True:
+            try:
+                r = todo.get_nowait()
+            except queue.Empty:
+                return
+            t1 = time.perf_counter()
+            try:
+                _res, wall, loop = refrun.run(r.case, fast=True, keep_outputs=False, cores=str(core),
+                                              timeout=3600)
+            except Exception as e:  # reported in the JSON line
+                with lock:
+                    errors.append(f"request {r.rid}: {str(e)[:200]}")
+                continue
+            with lock:
+                done.append((r.rid, time.perf_counter() - t1, loop))
+                print(f"[bench] cpu mix: request {r.rid} ({r.case.friction}, {r.case.cells/1e6:.1f}M cells, "
+                      f"{r.steps} steps) {wall:.1f} s on core {core}; {len(done)} of {len(reqs)} done",
+                      file=sys.stderr, flush=True)
+
+    threads = [threading.Thread(target=worker, args=(c,)) for c in cores]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t0
+    upd = sum(r.case.cells * r.steps for r in reqs if any(d[0] == r.rid for d in done))
+    mean_all = sum(r.case.cells * r.steps for r in allreq) / len(allreq)
+    gups = upd / wall / 1e9
+    out = {"metric": "served ruptures/s (reference CPU build, concurrent processes)",
+           "value": len(done) / wall, "unit": "ruptures/s", "n_gpus": 0, "steps": None, "warmup": 0,
+           "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 (rate-and-state: the f64 build)", "data": "synthetic",
+           "config": {"workload": f"served_stream_first_{len(reqs)}_of_2000_cpu", "requests": len(reqs),
+                      "processes": nproc, "cores": cores, "core_model": model, "host_cores": ncpu,
+                      "build": "reference Fortran, flang -O3 -ffast-math (-fdefault-real-8 for rate-and-state), "
+                               "serial, one process per core (taskset)"},
+           "wall_s": wall, "Gcell_updates_per_s": gups,
+           "mix_ruptures_per_s": gups * 1e9 / mean_all,
+           "mix_note": "the whole stream's rate at this cell-update throughput (mean work per request of "
+                       "the 2000 requests)",
+           "per_request_s": sorted(d[1] for d in done), "errors": errors}
     print(json.dumps(out))
 
 
@@ -284,6 +395,8 @@ def main():
         return bench_batched(a)
     if a.workload == "served":
         return bench_served(a)
+    if a.workload == "served-cpu":
+        return bench_served_cpu(a)
     import numpy as np
     from fd3d_tsn_amd import capi
     from fd3d_tsn_amd import cases as C
@@ -328,20 +441,23 @@ def main():
     # while the GPU computes the next
     outputs_on = None
     if not a.no_outputs_on:
-        chunk = max(1, min(32, a.steps // 4))
+        # at least 64 steps in chunks of 16: the last chunk's copy (the only
+        # one not hidden behind computing) is then a quarter of a chunk run
+        n_on, chunk = max(a.steps, 64), 16
         s.step(2, outputs=False)
         s.sync()
         t0 = time.perf_counter()
         peak = 0.0
-        for res in s.stream(a.steps, chunk):
-            peak = max(peak, float(np.max(np.abs(res["sliprateX"]))))
+        for res in s.stream(n_on, chunk, copy=False):
+            peak = max(peak, float(np.max(np.abs(res["sliprateX"][-1]))))
         el = time.perf_counter() - t0
         nxT, nzT = case.fault_shape
-        outputs_on = {"ms_per_step": el * 1e3 / a.steps, "chunk_steps": chunk,
+        outputs_on = {"ms_per_step": el * 1e3 / n_on, "steps": n_on, "chunk_steps": chunk,
                       "host_bytes_per_step": 5 * nxT * nzT * 4 + 3 * 4 * len(case.stations),
-                      "of_outputs_off": (el / a.steps) / (elapsed / a.steps),
+                      "of_outputs_off": (el / n_on) / (elapsed / a.steps),
                       "note": "every step's 5 output slabs and the station samples copied to pinned host "
-                              "memory on a copy stream, the host reading each chunk while the next computes"}
+                              "memory on a copy stream (fd3d_step_begin / fd3d_step_wait), the host reading "
+                              "each chunk while the next computes; timed until the last chunk is on the host"}
     step_bytes, nbulk, npml = s.step_bytes()
     tuning = s.get_tuning()
     fused = bool(tuning[0] & 4)
@@ -405,6 +521,21 @@ def main():
     if outputs_on is not None:
         out["ms_per_step_outputs_on"] = outputs_on["ms_per_step"]
         out["outputs_on"] = outputs_on
+    # the batched workload (64 uneven domains) beside it, same steps and warm-up
+    if not a.no_batched and a.workload == "large_sw":
+        print("[bench] batched leg: 64 uneven domains", file=sys.stderr, flush=True)
+        try:
+            bt = bench_batched(a, emit=False)
+            out["batched_64_domains"] = {
+                "ruptures_per_s": bt["value"], "ms_per_step": bt["ms_per_step"],
+                "Gcell_updates_per_s": bt["batch"]["Gcell_updates_per_s"], "hbm_frac": bt["batch"]["hbm_frac"],
+                "plain_back_to_back_ruptures_per_s": bt["plain_back_to_back"]["ruptures_per_s"],
+                "plain_back_to_back_ms_per_step_sum": bt["plain_back_to_back"]["ms_per_step_sum"],
+                "speedup_vs_plain": bt["speedup_vs_plain"], "cells": bt["config"]["cells"],
+                "steps_per_rupture": bt["config"]["steps_per_rupture"], "tz": bt["config"]["tz"],
+                "fused_members": bt["config"]["fused_members"]}
+        except Exception as e:  # reported, not fatal: the main number stands
+            out["batched_64_domains"] = {"error": str(e)[:300]}
     if cpu is not None:
         out["cpu_baseline"] = cpu
     print(json.dumps(out))

@@ -289,19 +289,23 @@ class Solver:
         res["seis"] = seis[:, :nsta, :]
         return res
 
-    def stream(self, nsteps: int, chunk: int = 32):
+    def stream(self, nsteps: int, chunk: int = 32, copy: bool = True):
         """Advance nsteps, yielding each chunk's outputs while the GPU
-        computes the next chunk (the overlapped output stream)."""
+        computes the next chunk (the overlapped output stream).  With
+        copy=False the yielded arrays are views of the pinned buffer, valid
+        until the generator resumes."""
         chunk = max(1, min(chunk, self.ring_steps // 2))
         left = nsteps
         inflight = 0
         while left > 0 or inflight:
+            # keep two chunks in flight: the one handed out last is done with
+            # once the generator resumes, so its buffer can take the next
             while left > 0 and inflight < 2:
                 n = min(chunk, left)
                 self.step_begin(n)
                 left -= n
                 inflight += 1
-            yield self.step_wait()
+            yield self.step_wait(copy=copy)
             inflight -= 1
 
     def read_outputs(self, nsteps: int) -> Dict[str, np.ndarray]:
