@@ -521,6 +521,30 @@ def main():
     if outputs_on is not None:
         out["ms_per_step_outputs_on"] = outputs_on["ms_per_step"]
         out["outputs_on"] = outputs_on
+    # the other single-rupture workloads beside it (same steps and warm-up):
+    # the rate-and-state rupture and the small parity rupture
+    if not a.no_batched and a.workload == "large_sw":
+        for wname, wcase in (("fvw_512x256x256", C.c3_fvw()), ("plain_sw_128x64x64", C.c1_plain_sw())):
+            try:
+                fault2, scal2 = (C.fault_fvw(wcase) if wcase.friction == "fvw" else (None, None))
+                with capi.Solver(wcase, fault=fault2, fvw_scalars=scal2) as s2:
+                    autotune.apply(s2)
+                    s2.step(a.warmup, outputs=False)
+                    s2.sync()
+                    t0 = time.perf_counter()
+                    s2.step(a.steps, outputs=False)
+                    s2.sync()
+                    el2 = (time.perf_counter() - t0) / a.steps
+                    b2, nb2, np2 = s2.step_bytes()
+                    t2 = s2.get_tuning()
+                moved2 = (MOVED_BULK_FUSED if t2[0] & 4 else MOVED_BULK) * nb2 + MOVED_PML * np2
+                out[wname] = {"ms_per_step": el2 * 1e3, "Gcell_updates_per_s": wcase.cells / el2 / 1e9,
+                              "hbm_frac_moved_bytes": moved2 / el2 / 1e9 / HBM_PEAK_GBS,
+                              "hbm_frac_convention": b2 / el2 / 1e9 / HBM_PEAK_GBS,
+                              "grid": [wcase.nxt, wcase.nyt, wcase.nzt], "friction": wcase.friction,
+                              "fspace": wcase.fspace, "tuning": list(t2)}
+            except Exception as e:  # reported, not fatal
+                out[wname] = {"error": str(e)[:300]}
     # the batched workload (64 uneven domains) beside it, same steps and warm-up
     if not a.no_batched and a.workload == "large_sw":
         print("[bench] batched leg: 64 uneven domains", file=sys.stderr, flush=True)

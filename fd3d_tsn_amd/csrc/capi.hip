@@ -807,13 +807,15 @@ static int stream_setup(fd3d_handle *h) {
   const size_t slab = (size_t)h->p.g.nxT * h->p.g.nzT;
   const size_t half = (size_t)h->cfg.ring_steps / 2;
   const size_t floats = half * FD3D_NOUT * slab + half * (size_t)std::max(1, h->cfg.nstations) * 3;
-  for (auto &c : h->chunk) {
-    void *p = nullptr;
-    HIPCHK(h, hipHostMalloc(&p, floats * sizeof(float) + sizeof(Ctl), hipHostMallocDefault));
-    c.host = static_cast<float *>(p);
-    c.ctl = reinterpret_cast<Ctl *>(c.host + floats);
-    HIPCHK(h, hipEventCreateWithFlags(&c.computed, hipEventDisableTiming));
-    HIPCHK(h, hipEventCreateWithFlags(&c.copied, hipEventDisableTiming));
+  for (auto &c : h->chunk) {   // (a call after a failed one completes what is missing)
+    if (!c.host) {
+      void *p = nullptr;
+      HIPCHK(h, hipHostMalloc(&p, floats * sizeof(float) + sizeof(Ctl), hipHostMallocDefault));
+      c.host = static_cast<float *>(p);
+      c.ctl = reinterpret_cast<Ctl *>(c.host + floats);
+    }
+    if (!c.computed) HIPCHK(h, hipEventCreateWithFlags(&c.computed, hipEventDisableTiming));
+    if (!c.copied) HIPCHK(h, hipEventCreateWithFlags(&c.copied, hipEventDisableTiming));
   }
   HIPCHK(h, hipStreamCreateWithFlags(&h->copy, hipStreamNonBlocking));
   return 0;
