@@ -93,3 +93,13 @@ def test_batch_member_fused_chunks():
     assert A.batch_member(small, None, 16) == (1 | 4, 16, 62, 16, 32)
     assert A.batch_member(small, (1, 8, 16, 8, 32), 8, fused=False) == (1, 8, 16, 8, 32)
     assert A.batch_member(C.c3_fvw(), (1, 8, 16, 8, 32), 8) == (1 | 4, 8, 128, 8, 32)   # free surface
+
+
+def test_tiled_pml_keeps_the_tile_width_bits(cache):
+    """When the tiled PML sweep measured faster, choose() adds mode bit 1 to
+    the interior shape it picked (128-wide tiles included)."""
+    case = C.c2_large_sw()
+    tab = {"vel": {"w8x32": 2.5, "16x32": 3.0}, "str": {"w8x16": 3.9, "16x32": 4.0},
+           "pml": {"flat": 3.0, "tiled": 2.9}}
+    json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
+    assert A.choose(case) == (1 | 2 | 8 | 16, 8, 32, 8, 16)
