@@ -17,9 +17,12 @@ of sizes and keeps the PML width and the free-surface setting.
 
 The fused velocity + stress sweep (mode bit 2) is timed too, at several rows per block; :func:`choose` takes it when it
 beats the best pair of separate sweeps by 5%, with rows per block from
-the problem's own geometry (:func:`fused_ny`).
+the problem's own geometry (:func:`fused_ny`), and in strips of the
+width measured fastest for the class (:func:`tune_strips`, mode bits 5-6).
 
 * :func:`tune` measures one problem and stores its table;
+* :func:`tune_strips` times the fused sweep's strip widths on a problem
+  whose class takes the fused sweep and adds them to its table;
 * :func:`choose` returns the best (tz, ny) of a problem's class, or the
   best ny for a given tz (the batch engine uses one tz for all members);
 * :func:`apply` sets a solver's shape from the cache (or leaves the
@@ -136,6 +139,51 @@ def tune(solver: "capi.Solver", steps: int = 3, warmup: int = 1, save: bool = Tr
     return table
 
 
+# Strip widths of the fused sweep's tile order and their mode bits 5-6
+# (fused.h strip_of_mode): tiles are walked in z within strips this many
+# x-tiles wide.  The best width depends on how the tile count splits over
+# the 8 XCDs: 1024x512x512 (16 x 47 tiles, 94 per XCD) ran 4.61 ms in
+# strips of 2 against 4.70 ms in strips of 4, 512x256x256 (8 x 24 tiles
+# in 4 y chunks, 96 per XCD) 0.656 ms in strips of 4 against 0.716 ms in
+# strips of 2.
+STRIPS = {4: 0, 2: 1 << 5, 8: 2 << 5}
+
+
+STRIP_MARGIN = 0.99
+
+
+def tune_strips(solver: "capi.Solver", steps: int = 5, warmup: int = 2, rounds: int = 3,
+                save: bool = True) -> Optional[Dict]:
+    """Time the fused sweep in each strip width of STRIPS at the shape
+    :func:`choose` gives the solver's class (a class that does not take
+    the fused sweep is left alone): `rounds` passes over the widths, the
+    best average launch of each, stored as the entry's "fused_strip"
+    {"width": ms}.  Results do not depend on the width."""
+    case = solver.case
+    t = choose(case)
+    if t is None or not t[0] & 4:
+        return None
+    before = solver.get_tuning()
+    res = {}
+    solver.step(warmup, outputs=False)
+    for _ in range(rounds):
+        for w, bits in STRIPS.items():
+            solver.set_tuning((t[0] & ~96) | bits, *t[1:])
+            solver.step(1, outputs=False)
+            solver.set_profiling(True)
+            solver.step(steps, outputs=False)
+            kt = solver.kernel_times()
+            solver.set_profiling(False)
+            ms = kt["fused_interior"][0] / kt["fused_interior"][1]
+            res[str(w)] = min(ms, res.get(str(w), ms))
+    solver.set_tuning(*before)
+    if save:
+        db = _load()
+        db.setdefault("entries", {}).setdefault(size_class(case), {})["fused_strip"] = res
+        _save(db)
+    return res
+
+
 def _best(tab: Dict[str, float], tz: Optional[int] = None) -> Optional[Tuple[bool, int, int, float]]:
     """Fastest (wide, tz, ny, ms) of a table; with tz given, among the
     64-wide shapes of that height only (the batch engine's tiles)."""
@@ -199,7 +247,15 @@ def choose(case: C.Case, tz: Optional[int] = None) -> Optional[Tuple[int, int, i
     if tz is None and fz:
         fms = min(fz.values())
         if fms < FUSED_MARGIN * (_best(e["vel"])[3] + _best(e["str"])[3]):
-            return (mode | 4, tv, fused_ny(case), ts, ns)
+            # another strip width than the library's 4 only when it
+            # measured at least 1% faster
+            fs = e.get("fused_strip")
+            strip = 0
+            if fs and "4" in fs:
+                w = min(fs, key=fs.get)
+                if fs[w] < STRIP_MARGIN * fs["4"]:
+                    strip = STRIPS[int(w)]
+            return (mode | 4 | strip, tv, fused_ny(case), ts, ns)
     return (mode, tv, nv, ts, ns)
 
 
@@ -256,8 +312,10 @@ def batch_plan(cases: Iterable[C.Case], fused: bool = True) -> Tuple[int, List[O
 
 
 def main():
-    """python -m fd3d_tsn_amd.autotune [batched|served N|single|large] : tune the
-    size classes of a workload on the GPU and update the cache."""
+    """python -m fd3d_tsn_amd.autotune [batched|served N|single|large] [N strips]: tune
+    the size classes of a workload on the GPU and update the cache (with
+    "strips", only the fused sweep's strip widths of the classes that take
+    it)."""
     import sys
     import time
     which = sys.argv[1] if len(sys.argv) > 1 else "batched"
@@ -271,6 +329,7 @@ def main():
         cs = [C.c1_plain_sw(), C.c2_large_sw(), C.c3_fvw()]
     else:
         cs = [C.c2_large_sw()]
+    strips_only = len(sys.argv) > 3 and sys.argv[3] == "strips"
     seen = set()
     for c in cs:
         k = size_class(c)
@@ -278,6 +337,15 @@ def main():
             continue
         seen.add(k)
         t0 = time.perf_counter()
+        if strips_only:
+            # the fused sweep's strip widths only, for classes already in
+            # the cache that take it
+            with capi.Solver(c.replace(ntfd=16, stations=[])) as s:
+                res = tune_strips(s)
+            if res:
+                print(f"[autotune] {k}: fused strips " + " ".join(f"{w}:{ms:.3f}" for w, ms in res.items()) +
+                      f" ({time.perf_counter()-t0:.1f} s)", flush=True)
+            continue
         with capi.Solver(c.replace(ntfd=8, stations=[])) as s:
             tab = tune(s)
         bv, bs = _best(tab["vel"]), _best(tab["str"])

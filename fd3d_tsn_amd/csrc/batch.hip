@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(fused::NT) k_batch_fused(BatchArgs A, int seg_
   const int kmax_s = D.p.g.nzt - D.p.g.nfs - (D.p.g.fspace ? 0 : 2);
   const int per = D.fgx * D.fgz, by = b / per;
   int bx, bz;
-  fused::strip_tile(b - by * per, D.fgx, D.fgz, bx, bz);
+  fused::strip_tile(b - by * per, D.fgx, D.fgz, fused::kStrip, bx, bz);
   fused::fused_body(D.p.g, D.p.c, D.p.f[0], D.p.f[1], D.p.ctl, bx, bz, by, D.nyf, kmax_s, L);
 }
 

@@ -1011,7 +1011,8 @@ int fd3d_download_source_bins(fd3d_handle *h, float *msrx, float *msrz, float *m
 int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str) {
   if (!h) return FD3D_E_INVALID;
   auto ok_tz = [](int tz) { return tz == 4 || tz == 8 || tz == 16; };
-  if (mode < 0 || mode > 31) return fail(h, FD3D_E_INVALID, "tuning mode must be 0..31");
+  if (mode < 0 || mode > 127) return fail(h, FD3D_E_INVALID, "tuning mode must be 0..127");
+  if ((mode & 96) && !(mode & 4)) return fail(h, FD3D_E_INVALID, "the strip width (bits 5, 6) needs bit 2");
   if ((mode & 24) && !(mode & 1)) return fail(h, FD3D_E_INVALID, "tile widths (bits 3, 4) need bit 0");
   if (((mode & 8) && tz_vel > 8) || ((mode & 16) && tz_str > 8))
     return fail(h, FD3D_E_INVALID, "128-wide tiles take tz 4 or 8");

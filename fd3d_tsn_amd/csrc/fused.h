@@ -85,15 +85,26 @@ static_assert(4 * NZP <= 64, "the x-ring fits one wavefront");
 // slightly more (1.42 against 1.36 times the algorithmic reads); equal on
 // 512x256x256.  In the batch engine's fused stage the step of 64 uneven
 // domains (96-320 cells a side) took 25.5 against 26.2 ms and 24.0 against
-// 24.5 ms (two pairs on one GPU).  Strips 2 tiles wide: 4.53 ms on
-// 1024x512x512 but 8% slower on 512x256x256; z-fastest order (1 tile
-// wide): 5.93 ms.
+// 24.5 ms (two pairs on one GPU).  z-fastest order (1 tile wide): 5.93
+// ms.  The width of a single problem's strips is a tuning choice (mode
+// bits 5-6, 4 by default), since how the tiles divide over the 8 XCDs
+// decides it: 1024x512x512 (16 x 47 tiles, 94 per XCD, one strip of 2
+// each) runs 4.61 ms in strips of 2 against 4.70 ms in strips of 4 and
+// 5.12 ms in strips of 3; 512x256x256 (8 x 24 tiles in 4 y chunks, 96 per
+// XCD) 0.656 ms in strips of 4 against 0.716 ms in strips of 2.  The batch
+// engine's fused stage keeps strips of 4.
 constexpr int kStrip = 4;
-// Tile (bx, bz) of number r in [0, gx * gz) in that order.
-__device__ __forceinline__ void strip_tile(int r, int gx, int gz, int &bx, int &bz) {
-  const int st = r / (kStrip * gz), w = min(kStrip, gx - st * kStrip), q = r - st * kStrip * gz;
-  bx = st * kStrip + q % w;
+// Tile (bx, bz) of number r in [0, gx * gz) in that order, strips `strip`
+// tiles wide.
+__device__ __forceinline__ void strip_tile(int r, int gx, int gz, int strip, int &bx, int &bz) {
+  const int st = r / (strip * gz), w = min(strip, gx - st * strip), q = r - st * strip * gz;
+  bx = st * strip + q % w;
   bz = q / w;
+}
+// Strip width of tuning mode bits 5-6.
+__host__ __device__ inline int strip_of_mode(int mode) {
+  const int code = (mode >> 5) & 3;
+  return code == 0 ? kStrip : code == 1 ? 2 : code == 2 ? 8 : 3;
 }
 
 typedef float f4 __attribute__((ext_vector_type(4)));

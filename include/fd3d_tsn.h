@@ -206,7 +206,9 @@ int fd3d_download_source_bins(fd3d_handle *h, float *msrx, float *msrz, float *m
  * per block; under a free surface the top two interior planes get their
  * stresses after the ghost velocities, in a small pass of their own); mode
  * bits 3 and 4 (with bit 0): velocity and stress tiles 128 wide instead of
- * 64 (tz in {4, 8}).  fd3d_create picks a default from the grid size (the
+ * 64 (tz in {4, 8}); mode bits 5-6 (with bit 2): the fused sweep walks its
+ * tiles in z within strips 4 (0), 2 (1), 8 (2) or 3 (3) x-tiles wide.
+ * fd3d_create picks a default from the grid size (the
  * fused sweep for grids whose 64 x 11 x-z tiles times interior rows reach
  * 8192, such as 256x256x256 and up);
  * fd3d_tsn_amd/autotune.py measures and caches the best.

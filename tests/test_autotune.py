@@ -103,3 +103,23 @@ def test_tiled_pml_keeps_the_tile_width_bits(cache):
            "pml": {"flat": 3.0, "tiled": 2.9}}
     json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
     assert A.choose(case) == (1 | 2 | 8 | 16, 8, 32, 8, 16)
+
+
+def test_fused_strip_width_only_when_one_percent_faster(cache):
+    """choose() keeps the library's strips of 4 tiles unless another width
+    measured at least 1% faster for the class, and then sets its mode bits
+    5-6 (2 tiles: 32, 8 tiles: 64) beside the fused sweep's bit 2."""
+    case = C.c2_large_sw()
+    base = {"vel": {"16x32": 3.0}, "str": {"16x32": 3.0}, "pml": {"flat": 3.0}, "fused": {"256": 4.6}}
+    ny = A.fused_ny(case)
+    for strips, bits in (({"4": 4.70, "2": 4.61, "8": 4.75}, 32),
+                         ({"4": 0.600, "2": 0.620, "8": 0.570}, 64),
+                         ({"4": 4.70, "2": 4.67, "8": 4.69}, 0),     # within 1%: strips of 4
+                         (None, 0)):
+        tab = dict(base)
+        if strips:
+            tab["fused_strip"] = strips
+        json.dump({"entries": {A.size_class(case): tab}}, open(cache, "w"))
+        assert A.choose(case) == (1 | 4 | bits, 16, ny, 16, 32), strips
+    # the batch engine's tile height: no fused sweep, no strip bits
+    assert A.choose(case, 16)[0] & 100 == 0

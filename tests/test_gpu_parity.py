@@ -189,7 +189,8 @@ def test_wide_tiles_bit_exact(tuning):
 def test_wide_tiles_reject_bad_shapes():
     case = C.tiny(friction="sw", fspace=True, seed=38)
     with capi.Solver(case) as s:
-        for bad in ((8, 8, 4, 8, 4), (9, 16, 4, 8, 4), (17, 8, 4, 16, 4), (32, 8, 4, 8, 4)):
+        for bad in ((8, 8, 4, 8, 4), (9, 16, 4, 8, 4), (17, 8, 4, 16, 4), (32, 8, 4, 8, 4), (65, 8, 4, 8, 4),
+                    (128, 8, 4, 8, 4)):
             with pytest.raises(capi.FD3DError) as e:
                 s.set_tuning(*bad)
             assert e.value.code == -1, bad
@@ -206,6 +207,10 @@ FUSED = [("fs_ny3", dict(nabc=5, n=(70, 17, 21), seed=33), 3, True),
          # 5 tiles across: a strip of 4 tiles and a narrower last strip
          ("fs_strips", dict(nabc=4, n=(300, 9, 25), seed=43), 8, True),
          ("free_strips", dict(nabc=4, n=(300, 9, 25), seed=44), 8, False),
+         # strips 2 (2, 2, 1), 3 (3, 2) and 8 (one strip of 5) tiles wide
+         ("fs_strips2", dict(nabc=4, n=(300, 9, 25), seed=45), 8, True, 5 | 32),
+         ("free_strips3", dict(nabc=4, n=(300, 9, 25), seed=46), 8, False, 5 | 96),
+         ("fs_strips8", dict(nabc=4, n=(300, 9, 25), seed=47), 8, True, 5 | 64),
          ("fs_tiledpml", dict(nabc=8, n=(70, 17, 21), seed=41), 8, True, 7),
          ("free_tiledpml", dict(nabc=8, n=(70, 17, 21), seed=42), 8, False, 7)]
 
