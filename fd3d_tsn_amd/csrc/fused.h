@@ -180,6 +180,12 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
   const bool int_xz = has && i >= g.nabc + 1 && i <= g.nxt - g.nabc && k >= g.nabc + 1 && k <= g.nzt - g.nfs;
   const bool core = int_xz && wv >= 2 && wv < 2 + ZC;
   const bool core_s = core && k <= kmax_s;   // stores new stresses
+  // In a tile the interior ends in (the last x or z tile of a grid whose
+  // interior is not a multiple of the tile), the stencils of the last
+  // interior column and plane reach 2 cells past it: stress planes and
+  // velocities beyond are never read, so they are not loaded.
+  const int xlim = g.nxt - g.nabc - i0 + 2, zlim = g.nzt - g.nfs - k0 + 2;
+  const bool need_v = in_xz && x <= xlim && z <= zlim;   // velocity source read
   const int t = (z + 2) * PW + PX0 + x;   // the lane's slot in a velocity plane
   const int sy = g.sy;
   const float c1 = c.c1, c2 = c.c2, dth = c.dth;
@@ -203,7 +209,7 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
 #define FUSED_VSRC(j, a, bb, cc)                                                      \
   do {                                                                              \
     a = bb = cc = 0.0f;                                                             \
-    if (in_xz && (j) >= 1 && (j) <= g.nyt) {                                        \
+    if (need_v && (j) >= 1 && (j) <= g.nyt) {                                       \
       const int q_ = qb + ((j) - 1) * sy;                                           \
       if (int_xz && (j) >= jlo && (j) <= jhi) { a = ld32(ou, q_); bb = ld32(ov, q_); cc = ld32(ow, q_); } \
       else { a = ld32(nu_, q_); bb = ld32(nv_, q_); cc = ld32(nw_, q_); }            \
@@ -227,7 +233,8 @@ __device__ __forceinline__ void fused_body(const Dims &g, const Coef &c, const F
       if (plane_chunk(m, tx, f, sl, px, pz)) {
         pslot[n] = sl;
         const int gi = i0 + px, gk = k0 + pz;
-        if (gk >= 1 && gk <= g.nzt && g.off0 + gi + 3 <= g.px) poff[n] = I3(g, gi, 1, gk);
+        if (gk >= 1 && gk <= g.nzt && g.off0 + gi + 3 <= g.px && px <= xlim && pz <= zlim)
+          poff[n] = I3(g, gi, 1, gk);
       }
       psrc[n] = f == 0 ? oxx : f == 1 ? oxy : f == 2 ? oxz : f == 3 ? oyz : ozz;
     }
