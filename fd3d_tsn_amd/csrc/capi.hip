@@ -75,6 +75,18 @@ int fused_rows(long long tiles, int nyi, int ncu) {
   return brows;
 }
 
+// Strip width of the fused sweep's tile order (mode bits 5-6): the 8 XCDs
+// take contiguous ranges of the tile numbers, and the measured optima are
+// the widths at which each XCD's range is exactly one strip (1024x512x512:
+// 16 x 47 tiles, 94 per XCD, strips of 2, 4.61 against 4.70 ms in strips of
+// 4; 512x256x256: 8 x 24 tiles in 4 y chunks, 96 per XCD, strips of 4).
+// Other grids keep strips of 4 (the autotuner measures the widths).
+int fused_strip_bits(int fgx, int fgz, int fgy) {
+  const long long n = (long long)fgx * fgz * fgy;
+  if (n % 8 != 0 || (n / 8) % fgz != 0) return 0;
+  return n / 8 / fgz == 2 ? 32 : 0;
+}
+
 Tuning default_tuning(const Dims &g, int ncu) {
   Tuning t{};
   t.mode = 1;
@@ -89,8 +101,8 @@ Tuning default_tuning(const Dims &g, int ncu) {
   t.ny_vel = t.ny_str = rows;
   const long long ftiles = (long long)((nx + 63) / 64) * ((nz + 10) / 11);
   if (ny > 0 && ftiles * ny >= kFusedTileRows) {
-    t.mode = 1 | 4;
     t.ny_vel = fused_rows(ftiles, ny, ncu > 0 ? ncu : 256);
+    t.mode = 1 | 4 | fused_strip_bits((nx + 63) / 64, (nz + 10) / 11, (ny + t.ny_vel - 1) / t.ny_vel);
   }
   return t;
 }

@@ -210,7 +210,8 @@ int fd3d_download_source_bins(fd3d_handle *h, float *msrx, float *msrz, float *m
  * tiles in z within strips 4 (0), 2 (1), 8 (2) or 3 (3) x-tiles wide.
  * fd3d_create picks a default from the grid size (the
  * fused sweep for grids whose 64 x 11 x-z tiles times interior rows reach
- * 8192, such as 256x256x256 and up);
+ * 8192, such as 256x256x256 and up, in strips of 2 tiles when that makes
+ * each of the 8 XCDs' tile ranges exactly one strip, else of 4);
  * fd3d_tsn_amd/autotune.py measures and caches the best.
  * Results do not depend on the shape (bit-identical). */
 int fd3d_set_tuning(fd3d_handle *h, int mode, int tz_vel, int ny_vel, int tz_str, int ny_str);

@@ -70,6 +70,8 @@ def test_large_sw_fused_graph_equals_separate_kernel_by_kernel():
     say(f"[fullsize] {case.name} {case.nxt}x{case.nyt}x{case.nzt}: fused (graph) against separate (kernel by kernel)")
     with capi.Solver(case) as a, capi.Solver(case) as b:
         assert a.get_tuning()[0] & 4, "the library's default on this grid is the fused sweep"
+        # 16 x 47 tiles, 94 per XCD: the default walks them in strips of 2
+        assert a.get_tuning()[0] & 96 == 32, a.get_tuning()
         assert autotune.apply(a), "the size class is in the tuning cache"
         ta = a.get_tuning()
         assert ta[0] & 4, f"the fused sweep is the benchmarked path (tuning {ta})"
