@@ -87,7 +87,8 @@ __device__ __forceinline__ void run_item(const BatchDomain &D, int stage, int b,
   constexpr int BLK = 64 * TZ;
   const Problem &P = D.p;
   const int t = ly * 64 + lx;
-  const FieldPair FP = field_pair(P.f[0], P.f[1], P.ctl);
+  // the fault-plane stages after advance read the step it has just counted
+  const FieldPair FP = stage > K_ADVANCE ? field_pair_done(P.f[0], P.f[1], P.ctl) : field_pair(P.f[0], P.f[1], P.ctl);
   const Fields &fo = FP.fo, &fn = FP.fn;
   switch (stage) {
     case K_VEL_INTERIOR:

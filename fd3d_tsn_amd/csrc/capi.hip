@@ -357,10 +357,12 @@ int check_ctl(fd3d_handle *h) { return fd3d_handle_check_ctl(h, h->stream); }
 // new velocities and old stresses, so vel_fault and pml_vel may run before
 // the fused interior sweep, which needs their new velocities.
 void drop_graph(fd3d_handle *h) {
-  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-  if (h->graph) (void)hipGraphDestroy(h->graph);
-  h->gexec = nullptr;
-  h->graph = nullptr;
+  for (int n = 0; n < 3; ++n) {
+    if (h->gexec[n]) (void)hipGraphExecDestroy(h->gexec[n]);
+    if (h->graph[n]) (void)hipGraphDestroy(h->graph[n]);
+    h->gexec[n] = nullptr;
+    h->graph[n] = nullptr;
+  }
 }
 
 void build_seq(fd3d_handle *h) {
@@ -380,12 +382,12 @@ void build_seq(fd3d_handle *h) {
   }
   h->seq.push_back(K_STR_FAULT); h->seq.push_back(K_PML_STR);
   if (!p.g.fspace) h->seq.push_back(K_FRES);
+  h->seq.push_back(K_ADVANCE);
   h->seq.push_back(K_TRACTION);
   if (!p.g.fspace) h->seq.push_back(K_TRACTION_MIRROR);
   h->seq.push_back(K_TABS); h->seq.push_back(K_FRICTION); h->seq.push_back(K_OUTPUTS);
   if (p.rates.on) h->seq.push_back(K_RATES);
   if (p.fv.nucleation) h->seq.push_back(K_NUCLEATION);
-  h->seq.push_back(K_ADVANCE);
 }
 
 }  // namespace
@@ -723,42 +725,105 @@ static int half_step_group(KernelId id, bool small) {
 // disjoint writes, neither reads what the other writes).
 constexpr long long kOverlapCells = 4LL << 20;
 
-// One step of h->seq captured into a graph on h->stream.  Kernel arguments
-// are fixed for the life of the handle (the wavefield set and the time
-// come from the device step counter), so one graph serves every step.  In
-// a run of kernels of the same half-step group (half_step_group) the first
-// goes on the main stream and the rest on the side stream (fork and join
-// by events), so they run side by side.
-static int capture_step(fd3d_handle *h) {
+// The step captured into three graphs on h->stream (kernel arguments are
+// fixed for the life of the handle: the wavefield set and the time come from
+// the device step counter, so the graphs serve every step):
+//   head    the step up to and including advance;
+//   rotated the previous step's fault-plane kernels (traction .. nucleation)
+//           and then vel_fault on the side stream, beside this step's other
+//           velocity sweeps (pml_vel, and vel_interior when the interior
+//           runs as two sweeps) on the main stream; then the rest of the
+//           step up to advance;
+//   tail    the fault-plane kernels of the last step.
+// The fault-plane kernels read the finished step's new set and the fault
+// arrays; the velocity sweeps read the same set and write the other one
+// and the PML splits, so the two groups share no written data.  vel_fault
+// reads what friction wrote and follows it on the side stream.  In a run
+// of kernels of the same half-step group (half_step_group) the first goes
+// on the main stream and the rest on the side stream (fork and join by
+// events), so they run side by side.
+static int capture_graphs(fd3d_handle *h) {
   const bool small = (long long)h->cfg.nxt * h->cfg.nyt * h->cfg.nzt < kOverlapCells;
-  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  auto abort_capture = [&](const std::string &what, hipError_t e) {
+  const std::vector<KernelId> &seq = h->seq;
+  const size_t nadv = (size_t)(std::find(seq.begin(), seq.end(), K_ADVANCE) - seq.begin());
+  if (nadv >= seq.size()) return fail(h, FD3D_E_STATE, "step sequence without advance");
+  const std::vector<KernelId> pre(seq.begin(), seq.begin() + nadv + 1), chain(seq.begin() + nadv + 1, seq.end());
+  size_t nlead = 0;   // leading velocity sweeps of the step
+  while (nlead < pre.size() && (pre[nlead] == K_VEL_INTERIOR || pre[nlead] == K_VEL_FAULT || pre[nlead] == K_PML_VEL))
+    ++nlead;
+  for (int kind = 0; kind < 3; ++kind) {
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    std::string what;
+    auto on = [&](KernelId id, hipStream_t st) {
+      if (e == hipSuccess && (e = launch(id, h->p, st)) != hipSuccess) what = kKernelNames[id];
+    };
+    auto ev = [&](hipEvent_t evt, hipStream_t rec, hipStream_t wait) {
+      if (e == hipSuccess && (e = hipEventRecord(evt, rec)) == hipSuccess) e = hipStreamWaitEvent(wait, evt, 0);
+      if (e != hipSuccess && what.empty()) what = "fork/join";
+    };
+    // kernels q[from..] with the half-step groups side by side
+    auto grouped = [&](const std::vector<KernelId> &q, size_t from) {
+      for (size_t n = from; n < q.size() && e == hipSuccess;) {
+        size_t m = n + 1;
+        const int grp = half_step_group(q[n], small);
+        while (grp && m < q.size() && half_step_group(q[m], small) == grp) ++m;
+        if (m - n > 1) ev(h->fork, h->stream, h->side);
+        on(q[n], h->stream);
+        for (size_t r = n + 1; r < m; ++r) on(q[r], h->side);
+        if (m - n > 1) ev(h->join, h->side, h->stream);
+        n = m;
+      }
+    };
+    if (kind == 0) {
+      grouped(pre, 0);
+    } else if (kind == 1) {
+      ev(h->fork, h->stream, h->side);
+      for (KernelId id : chain) on(id, h->side);
+      on(K_VEL_FAULT, h->side);
+      for (size_t n = 0; n < nlead; ++n)
+        if (pre[n] != K_VEL_FAULT) on(pre[n], h->stream);
+      ev(h->join, h->side, h->stream);
+      grouped(pre, nlead);
+    } else {
+      for (KernelId id : chain) on(id, h->stream);
+    }
     hipGraph_t g = nullptr;
-    (void)hipStreamEndCapture(h->stream, &g);
-    if (g) (void)hipGraphDestroy(g);
-    return fail(h, FD3D_E_HIP, "capture " + what + ": " + hipGetErrorString(e));
-  };
-  const std::vector<KernelId> &q = h->seq;
-  for (size_t n = 0; n < q.size();) {
-    size_t m = n + 1;
-    const int grp = half_step_group(q[n], small);
-    while (grp && m < q.size() && half_step_group(q[m], small) == grp) ++m;
-    hipError_t e;
-    if (m - n > 1) {
-      if ((e = hipEventRecord(h->fork, h->stream)) != hipSuccess) return abort_capture("fork", e);
-      if ((e = hipStreamWaitEvent(h->side, h->fork, 0)) != hipSuccess) return abort_capture("fork", e);
+    const hipError_t ee = hipStreamEndCapture(h->stream, &g);
+    if (e != hipSuccess || ee != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      drop_graph(h);
+      return fail(h, FD3D_E_HIP, "capture " + what + ": " + hipGetErrorString(e != hipSuccess ? e : ee));
     }
-    if ((e = launch(q[n], h->p, h->stream)) != hipSuccess) return abort_capture(kKernelNames[q[n]], e);
-    for (size_t r = n + 1; r < m; ++r)
-      if ((e = launch(q[r], h->p, h->side)) != hipSuccess) return abort_capture(kKernelNames[q[r]], e);
-    if (m - n > 1) {
-      if ((e = hipEventRecord(h->join, h->side)) != hipSuccess) return abort_capture("join", e);
-      if ((e = hipStreamWaitEvent(h->stream, h->join, 0)) != hipSuccess) return abort_capture("join", e);
+    h->graph[kind] = g;
+    const hipError_t ei = hipGraphInstantiate(&h->gexec[kind], g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      drop_graph(h);
+      return fail(h, FD3D_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
     }
-    n = m;
   }
-  HIPCHK(h, hipStreamEndCapture(h->stream, &h->graph));
-  HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+// Queue n steps on h->stream: kernel by kernel with events when profiling,
+// else head, n-1 rotated and the tail graph.
+static int run_steps(fd3d_handle *h, int n) {
+  if (n <= 0) return 0;
+  if (h->prof) {
+    for (int s = 0; s < n; ++s)
+      for (KernelId id : h->seq) {
+        int rc = launch_one(h, id);
+        if (rc) return rc;
+      }
+    return 0;
+  }
+  if (!h->gexec[0]) {
+    int rc = capture_graphs(h);
+    if (rc) return rc;
+  }
+  HIPCHK(h, hipGraphLaunch(h->gexec[0], h->stream));
+  for (int s = 1; s < n; ++s) HIPCHK(h, hipGraphLaunch(h->gexec[1], h->stream));
+  HIPCHK(h, hipGraphLaunch(h->gexec[2], h->stream));
   return 0;
 }
 
@@ -775,21 +840,8 @@ int fd3d_step(fd3d_handle *h, int nsteps, float *out, float *seis) {
   while (done < nsteps) {
     const int pos = h->steps % ring;
     const int chunk = std::min(nsteps - done, ring - pos);
-    if (h->prof) {
-      // kernel by kernel, with events around each launch
-      for (int s = 0; s < chunk; ++s) {
-        for (KernelId id : h->seq) {
-          int rc = launch_one(h, id);
-          if (rc) return rc;
-        }
-      }
-    } else {
-      if (!h->gexec) {
-        int rc = capture_step(h);
-        if (rc) return rc;
-      }
-      for (int s = 0; s < chunk; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
-    }
+    int rc = run_steps(h, chunk);
+    if (rc) return rc;
     if (out) {
       HIPCHK(h, hipMemcpyAsync(out + (size_t)done * FD3D_NOUT * slab, h->p.out.ring + (size_t)pos * FD3D_NOUT * slab,
                                (size_t)chunk * FD3D_NOUT * slab * sizeof(float), hipMemcpyDeviceToHost, h->stream));
@@ -854,14 +906,7 @@ int fd3d_step_begin(fd3d_handle *h, int nsteps) {
   const int ring = h->cfg.ring_steps;
   c.first = h->steps + 1;
   c.n = nsteps;
-  if (h->prof) {
-    for (int s = 0; s < nsteps; ++s)
-      for (KernelId id : h->seq)
-        if ((rc = launch_one(h, id))) return rc;
-  } else {
-    if (!h->gexec && (rc = capture_step(h))) return rc;
-    for (int s = 0; s < nsteps; ++s) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
-  }
+  if ((rc = run_steps(h, nsteps))) return rc;
   HIPCHK(h, hipEventRecord(c.computed, h->stream));
   HIPCHK(h, hipStreamWaitEvent(h->copy, c.computed, 0));
   const size_t slab = (size_t)h->p.g.nxT * h->p.g.nzT;

@@ -195,14 +195,19 @@ struct Problem {
 };
 
 // In the order the stages run within a step (the batch engine launches its
-// stages in this order).
+// stages in this order).  The step counter advances once the wavefield of
+// the step is complete (K_ADVANCE); the fault-plane stages after it read
+// that step's new set and time, and touch neither the wavefield nor
+// anything the next step's velocity sweeps read, so a single problem runs
+// them beside the next step's PML and interior velocity sweeps.
 enum KernelId {
   K_VEL_INTERIOR = 0, K_VEL_FAULT, K_PML_VEL,
   K_FUSED,             // interior velocities and stresses in one pass
   K_FUVW_UV, K_FUVW_W,
   K_STR_TOP,           // with the fused pass and a free surface: stresses of the top two interior planes
   K_STR_INTERIOR, K_STR_FAULT, K_PML_STR, K_FRES,
-  K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_RATES, K_NUCLEATION, K_ADVANCE,
+  K_ADVANCE,
+  K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_RATES, K_NUCLEATION,
   K_COUNT
 };
 extern const char *kKernelNames[K_COUNT];

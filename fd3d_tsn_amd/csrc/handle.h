@@ -33,11 +33,15 @@ struct fd3d_handle {
   std::vector<hipEvent_t> pool;
   double kms[K_COUNT] = {0};
   long kcount[K_COUNT] = {0};
-  // step sequence, and one step of it captured in a HIP graph (replayed by
-  // fd3d_step when profiling is off; rebuilt when the sequence changes)
+  // step sequence, and three graphs of it (replayed by fd3d_step when
+  // profiling is off; rebuilt when the sequence changes): a run of n steps
+  // is the head graph (a step up to advance), n-1 rotated graphs (the
+  // previous step's fault-plane kernels beside this step's velocity
+  // sweeps, then this step up to advance) and the tail graph (the last
+  // step's fault-plane kernels)
   std::vector<KernelId> seq;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
   // second stream and fork/join events of the captured step: the kernels of
   // a half-step that write disjoint cells run side by side
   hipStream_t side = nullptr;

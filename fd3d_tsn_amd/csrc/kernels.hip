@@ -1,11 +1,14 @@
 // Kernels of one FD3D_TSN time step for a single problem: one launch per
 // stage, each a thin wrapper around the stage bodies of stages.h.
 //
-// Step structure (one stream, kernels in order):
+// Step structure (kernels in order):
 //   velocity : vel_interior | vel_fault | pml_vel | (fuvw_uv, fuvw_w)
 //   stress   : str_interior | str_fault | pml_str | (fres)
-//   fault BC : traction | (traction_mirror) | tabs | friction | outputs | (nucleation)
 //   advance  : the step counter (which also selects the wavefield set)
+//   fault BC : traction | (traction_mirror) | tabs | friction | outputs | (nucleation)
+// The fault-plane kernels after advance read the step just counted (its
+// new set: field_pair_done); capi.hip runs them beside the next step's
+// velocity sweeps.
 // Kernels in parentheses run only with a free surface / rate-and-state
 // nucleation.  vel_fault performs the v mirror v(:,nyt,:) = v(:,nyt-1,:)
 // for every column before pml_vel updates the PML columns, as bnd2d does
@@ -17,8 +20,8 @@ namespace fd3d {
 
 const char *kKernelNames[K_COUNT] = {
     "vel_interior", "vel_fault", "pml_vel", "fused_interior", "fuvw_uv", "fuvw_w", "str_top",
-    "str_interior", "str_fault", "pml_str", "fres",
-    "traction", "traction_mirror", "tabs", "friction", "outputs", "rates", "nucleation", "advance"};
+    "str_interior", "str_fault", "pml_str", "fres", "advance",
+    "traction", "traction_mirror", "tabs", "friction", "outputs", "rates", "nucleation"};
 
 namespace {
 
@@ -27,6 +30,10 @@ constexpr int BY = 4;  // lane rows of the 64 x 4 blocks of the 2D stages
 #define B2 blockIdx.x, blockIdx.y, threadIdx.x, threadIdx.y
 #define PAIR                                     \
   const FieldPair FP = field_pair(f0, f1, ctl); \
+  const Fields &fo = FP.fo, &fn = FP.fn
+// the fault-plane kernels after advance: the step just counted
+#define PAIR_DONE                                     \
+  const FieldPair FP = field_pair_done(f0, f1, ctl); \
   const Fields &fo = FP.fo, &fn = FP.fn
 
 __global__ void __launch_bounds__(256) k_vel_interior(Dims g, Coef c, Fields f0, Fields f1, const Ctl *ctl) {
@@ -74,7 +81,7 @@ __global__ void __launch_bounds__(256) k_fres(Dims g, Fields f0, Fields f1, cons
   fres_body<BY>(g, fo, fn, B2);
 }
 __global__ void __launch_bounds__(256) k_traction(Dims g, Coef c, Fields f0, Fields f1, Fault fl, const Ctl *ctl) {
-  PAIR;
+  PAIR_DONE;
   traction_body<BY>(g, c, fo, fn, fl, B2);
 }
 __global__ void __launch_bounds__(256) k_traction_mirror(Dims g, Fault fl) {
@@ -83,13 +90,13 @@ __global__ void __launch_bounds__(256) k_traction_mirror(Dims g, Fault fl) {
 __global__ void __launch_bounds__(256) k_tabs(Dims g, Fault fl) { tabs_body<BY>(g, fl, B2); }
 __global__ void __launch_bounds__(256) k_friction(Dims g, Coef c, Fields f0, Fields f1, Fault fl, FvwScalars fv,
                                                   const Ctl *ctl) {
-  PAIR;
+  PAIR_DONE;
   friction_body<BY>(g, c, fo, fn, fl, fv, ctl, B2);
 }
 // The last block row records the stations.
 __global__ void __launch_bounds__(256) k_outputs(Dims g, Fields f0, Fields f1, Fault fl, FvwScalars fv, Outputs o,
                                                  Ctl *ctl) {
-  PAIR;
+  PAIR_DONE;
   if (blockIdx.y == gridDim.y - 1) {
     stations_body(g, fo, fn, o, ctl, blockIdx.x * 256 + threadIdx.y * BX + threadIdx.x);
     return;
@@ -103,12 +110,13 @@ __global__ void k_advance(Ctl *ctl) { advance_body(ctl); }
 __global__ void __launch_bounds__(1024) k_rates(Dims g, Coef c, Fields f0, Fields f1, Outputs o, Rates R,
                                                 const Ctl *ctl) {
   __shared__ double red[1024];
-  PAIR;
+  PAIR_DONE;
   (void)fo;
   rates_body<1024>(g, c, fn, o, R, ctl, blockIdx.x, threadIdx.x, red);
 }
 
 #undef PAIR
+#undef PAIR_DONE
 #undef B2
 
 __global__ void k_fill(float *p, float v, long long n) {

@@ -12,8 +12,10 @@
 // new state into the other (fn), so no stage overwrites a value another
 // stage of the same step still has to read, whatever the order of blocks.
 // Which set is which follows from the parity of the step counter
-// Ctl::it, which only the final stage of a step (advance_body) changes:
-// the state after step n is in set n & 1.  lam, mu and d are shared.
+// Ctl::it, which only advance_body changes, once the step's wavefield is
+// complete: the state after step n is in set n & 1.  The fault-plane
+// stages after it (traction .. nucleation) read the step just counted
+// (field_pair_done, time it * dt).  lam, mu and d are shared.
 //
 // Arithmetic contract: every update evaluates the reference's expression
 // with the same operands in the same left-to-right association, built with
@@ -596,7 +598,7 @@ __device__ inline void friction_body(const Dims &g, const Coef &c, const Fields 
   const int p = I2(g, i, k);
   const int q = I3(g, i, g.nyt, k), sz = g.sz;
   const float dt = c.dt;
-  const float time = (float)(ctl->it + 1) * dt;   // time at the end of this step
+  const float time = (float)ctl->it * dt;   // time at the end of this step (counted by advance)
   const float rup_tresh = 1.e-3f;
   const float *u = fn.u, *w = fn.w;
   // ---- Z-node
@@ -676,7 +678,7 @@ __device__ inline void friction_body(const Dims &g, const Coef &c, const Fields 
 // ---------------------------------------------------------------------------
 template <int BY>
 __device__ inline void outputs_body(const Dims &g, const Fields &fo, const Fields &fn, const Fault &fl, const FvwScalars &fv, const Outputs &o, Ctl *ctl, int gbx, int gby, int lx, int ly) {
-  const int it = ctl->it + 1;   // this step, 1-based
+  const int it = ctl->it;   // this step, 1-based (counted by advance)
   const int slot = (it - 1) % o.ring_steps;
   const int ks = g.nzt - g.nfs;
   const long long slab = (long long)g.nxT * g.nzT;
@@ -731,11 +733,11 @@ __device__ inline void outputs_body(const Dims &g, const Fields &fo, const Field
 template <int NT>
 __device__ inline void rates_body(const Dims &g, const Coef &c, const Fields &fn, const Outputs &o, const Rates &R,
                                   const Ctl *ctl, int item, int t, double *red) {
-  const int it1 = ctl->it + 1;   // this step, 1-based
+  const int it1 = ctl->it;   // this step, 1-based (counted by advance)
   const int kbin = (int)(((float)(it1 - 1) * c.dt) / R.dtseis) + 1;
   if (kbin < 1 || kbin > R.nbins) return;   // the reference reports CHYBA and writes out of range
   const long long slab = (long long)g.nxT * g.nzT;
-  const float *srx = o.ring + (long long)(ctl->it % o.ring_steps) * 5 * slab;
+  const float *srx = o.ring + (long long)((it1 - 1) % o.ring_steps) * 5 * slab;
   const float *srz = srx + slab;
   const float per = R.dtseis / c.dt;
   if (item == 0) {
@@ -784,8 +786,8 @@ __device__ inline void rates_body(const Dims &g, const Coef &c, const Fields &fn
   R.msrz[q] = (float)((double)R.msrz[q] + (double)sz / div);
 }
 
-// Last stage of a step: the step counter (and with it the buffer parity)
-// advances once every stage of the step has finished.
+// The step counter (and with it the buffer parity) advances once every
+// wavefield stage of the step has finished; the fault-plane stages follow.
 __device__ inline void advance_body(Ctl *ctl) { ctl->it += 1; }
 
 // The old (fo) and new (fn) wavefield sets of the step in progress.
@@ -795,6 +797,11 @@ struct FieldPair {
 __device__ inline FieldPair field_pair(const Fields &f0, const Fields &f1, const Ctl *ctl) {
   return (ctl->it & 1) ? FieldPair{f1, f0} : FieldPair{f0, f1};
 }
+// The same pair for the step advance_body has just counted (the stages
+// after it).
+__device__ inline FieldPair field_pair_done(const Fields &f0, const Fields &f1, const Ctl *ctl) {
+  return ((ctl->it - 1) & 1) ? FieldPair{f1, f0} : FieldPair{f0, f1};
+}
 
 // Station samples of one step into the device ring (fd3d_theo.f90:456-467),
 // one thread per station.
@@ -802,7 +809,7 @@ __device__ inline void stations_body(const Dims &g, const Fields &fo, const Fiel
                                      const Ctl *ctl, int s) {
   (void)fo;
   if (s >= o.nsta) return;
-  const int slot = ctl->it % o.ring_steps;
+  const int slot = (ctl->it - 1) % o.ring_steps;   // the step counted by advance
   const int q = I3(g, o.sta[3 * s], o.sta[3 * s + 1], o.sta[3 * s + 2]);
   float *dst = o.seis + ((long long)slot * o.nsta + s) * 3;
   dst[0] = fn.u[q];
@@ -813,7 +820,7 @@ __device__ inline void stations_body(const Dims &g, const Fields &fo, const Fiel
 // TPV104 smooth nucleation (fd3d_theo.f90:515-542), FP64.
 template <int BY>
 __device__ inline void nucleation_body(const Dims &g, const Coef &c, const Fault &fl, const FvwScalars &fv, const Ctl *ctl, int gbx, int gby, int lx, int ly) {
-  const double time = (double)((float)(ctl->it + 1) * c.dt);
+  const double time = (double)((float)ctl->it * c.dt);   // the step counted by advance
   const double t = time - (double)c.dt / 2.0;
   if (!(t <= fv.TT2)) return;
   const int i = g.nabc - 1 + gbx * BX + lx;
