@@ -325,8 +325,12 @@ struct fd3d_batch {
   int fused_items = 8;   // least fused work items per CU for the fused stage
   int fixed_dev = -1;    // fd3d_batch_create_static: tables in the device's fixed scratch, else -1
   long long wait_ticks = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // staged mode: head, rotated and tail graphs of a step (see capture_staged_t);
+  // persistent mode: graph 0 only
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
+  hipStream_t side = nullptr;               // the captured fork's second stream
+  hipEvent_t fork = nullptr, join = nullptr;
   struct Ev { hipEvent_t a, b; int n; };
   std::vector<Ev> evs;
   std::vector<hipEvent_t> pool;
@@ -437,30 +441,88 @@ template <int TZ, bool FVW>
 int capture_t(fd3d_batch *b) {
   BatchArgs A{b->d_seg, b->d_dom, b->d_cnt, (int)b->segs.size(), b->total, b->wait_ticks};
   const dim3 blk(64, TZ);
-  b->launches = 0;
-  if (b->mode == FD3D_BATCH_PERSISTENT) {
-    const int nc = (int)b->mem.size() * K_COUNT;
-    hipLaunchKernelGGL(k_batch_reset, dim3(cdiv(std::max(nc, 1), 256)), dim3(256), 0, b->stream, b->d_cnt, nc);
-    hipLaunchKernelGGL((k_batch_persistent<TZ, FVW>), dim3(b->workers), blk, 0, b->stream, A);
-    b->launches = 2;
-  } else {
-    for (int s = 0; s < K_COUNT; ++s) {
-      const int n = b->stage_item0[s + 1] - b->stage_item0[s];
-      if (n <= 0) continue;
-      if (s == K_FUSED)
-        hipLaunchKernelGGL(k_batch_fused, dim3(n), dim3(fused::LX, fused::NWV), 0, b->stream, A, b->stage_seg0[s],
-                           b->stage_seg0[s + 1] - 1, b->stage_item0[s], n);
-      else
-        launch_stage_rt<TZ, FVW>(s, n, b->stream, A, b->stage_seg0[s], b->stage_seg0[s + 1] - 1, b->stage_item0[s]);
-      ++b->launches;
-    }
-  }
+  const int nc = (int)b->mem.size() * K_COUNT;
+  hipLaunchKernelGGL(k_batch_reset, dim3(cdiv(std::max(nc, 1), 256)), dim3(256), 0, b->stream, b->d_cnt, nc);
+  hipLaunchKernelGGL((k_batch_persistent<TZ, FVW>), dim3(b->workers), blk, 0, b->stream, A);
+  b->launches = 2;
   return 0;
 }
 
+// Staged mode: one launch per stage covering every member, captured as
+//   head    a step up to and including advance;
+//   rotated the previous step's fault-plane stages (traction .. nucleation)
+//           and then vel_fault on the side stream, beside vel_interior and
+//           pml_vel; then the rest of the step up to advance;
+//   tail    the fault-plane stages of the last step.
+// Within a step vel_fault runs beside vel_interior and pml_vel, and
+// str_fault beside str_top, str_interior and pml_str: each group reads the
+// same old data and writes disjoint cells.  The fault-plane stages read the
+// finished step's new set and the fault arrays, which the velocity sweeps
+// of the next step neither write nor read (the kernels after advance take
+// the step it has counted).  A run of n steps replays head, n-1 rotated,
+// tail.
+template <int TZ, bool FVW>
+int capture_staged_t(fd3d_batch *b, int kind) {
+  BatchArgs A{b->d_seg, b->d_dom, b->d_cnt, (int)b->segs.size(), b->total, b->wait_ticks};
+  int launches = 0;
+  auto stage = [&](int s, hipStream_t st) {
+    const int n = b->stage_item0[s + 1] - b->stage_item0[s];
+    if (n <= 0) return;
+    if (s == K_FUSED)
+      hipLaunchKernelGGL(k_batch_fused, dim3(n), dim3(fused::LX, fused::NWV), 0, st, A, b->stage_seg0[s],
+                         b->stage_seg0[s + 1] - 1, b->stage_item0[s], n);
+    else
+      launch_stage_rt<TZ, FVW>(s, n, st, A, b->stage_seg0[s], b->stage_seg0[s + 1] - 1, b->stage_item0[s]);
+    ++launches;
+  };
+  hipError_t e = hipSuccess;
+  auto fork = [&]() {
+    if (e == hipSuccess) e = hipEventRecord(b->fork, b->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->fork, 0);
+  };
+  auto join = [&]() {
+    if (e == hipSuccess) e = hipEventRecord(b->join, b->side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join, 0);
+  };
+  static const int chain[] = {K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_RATES, K_NUCLEATION};
+  if (kind == 2) {
+    for (int s : chain) stage(s, b->stream);
+  } else {
+    fork();
+    if (kind == 1)
+      for (int s : chain) stage(s, b->side);
+    stage(K_VEL_FAULT, b->side);
+    stage(K_VEL_INTERIOR, b->stream);
+    stage(K_PML_VEL, b->stream);
+    join();
+    stage(K_FUSED, b->stream);
+    stage(K_FUVW_UV, b->stream);
+    stage(K_FUVW_W, b->stream);
+    fork();
+    stage(K_STR_FAULT, b->side);
+    stage(K_STR_TOP, b->stream);
+    stage(K_STR_INTERIOR, b->stream);
+    stage(K_PML_STR, b->stream);
+    join();
+    stage(K_FRES, b->stream);
+    stage(K_ADVANCE, b->stream);
+  }
+  if (e != hipSuccess) return bfail(b, FD3D_E_HIP, std::string("batch graph fork/join: ") + hipGetErrorString(e));
+  if (kind == 1) b->launches = launches;   // a step in a run: every stage once
+  return 0;
+}
+static_assert(K_ADVANCE + 8 == K_COUNT && K_NUCLEATION == K_COUNT - 1,
+              "capture_staged_t names every stage: advance, then the seven fault-plane stages");
+
+void drop_graphs(fd3d_batch *b) {
+  for (int n = 0; n < 3; ++n) {
+    if (b->exec[n]) { (void)hipGraphExecDestroy(b->exec[n]); b->exec[n] = nullptr; }
+    if (b->graph[n]) { (void)hipGraphDestroy(b->graph[n]); b->graph[n] = nullptr; }
+  }
+}
+
 int rebuild(fd3d_batch *b) {
-  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
-  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  drop_graphs(b);
   std::vector<BatchDomain> doms;
   b->fvw = false;
   bool fused = b->mode == FD3D_BATCH_STAGED;
@@ -519,22 +581,45 @@ int rebuild(fd3d_batch *b) {
     }
     if (rc) { b->dirty = true; return rc; }
   }
-  BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  switch (b->tz * 2 + (b->fvw ? 1 : 0)) {
-    case 8: rc = capture_t<4, false>(b); break;
-    case 9: rc = capture_t<4, true>(b); break;
-    case 16: rc = capture_t<8, false>(b); break;
-    case 17: rc = capture_t<8, true>(b); break;
-    case 32: rc = capture_t<16, false>(b); break;
-    default: rc = capture_t<16, true>(b); break;
+  const int kinds = b->mode == FD3D_BATCH_PERSISTENT ? 1 : 3;
+  for (int kind = 0; kind < kinds; ++kind) {
+    BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int key = b->tz * 2 + (b->fvw ? 1 : 0);
+    if (b->mode == FD3D_BATCH_PERSISTENT) {
+      switch (key) {
+        case 8: rc = capture_t<4, false>(b); break;
+        case 9: rc = capture_t<4, true>(b); break;
+        case 16: rc = capture_t<8, false>(b); break;
+        case 17: rc = capture_t<8, true>(b); break;
+        case 32: rc = capture_t<16, false>(b); break;
+        default: rc = capture_t<16, true>(b); break;
+      }
+    } else {
+      switch (key) {
+        case 8: rc = capture_staged_t<4, false>(b, kind); break;
+        case 9: rc = capture_staged_t<4, true>(b, kind); break;
+        case 16: rc = capture_staged_t<8, false>(b, kind); break;
+        case 17: rc = capture_staged_t<8, true>(b, kind); break;
+        case 32: rc = capture_staged_t<16, false>(b, kind); break;
+        default: rc = capture_staged_t<16, true>(b, kind); break;
+      }
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      drop_graphs(b);
+      b->dirty = true;
+      return rc ? rc : bfail(b, FD3D_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    b->graph[kind] = g;
+    e = hipGraphInstantiate(&b->exec[kind], g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graphs(b);
+      b->dirty = true;
+      return bfail(b, FD3D_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
   }
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(b->stream, &g);
-  if (rc) { if (g) (void)hipGraphDestroy(g); b->dirty = true; return rc; }
-  if (e != hipSuccess) { b->dirty = true; return bfail(b, FD3D_E_HIP, std::string("graph capture: ") + hipGetErrorString(e)); }
-  b->graph = g;
-  e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
-  if (e != hipSuccess) { b->dirty = true; return bfail(b, FD3D_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
   return 0;
 }
 
@@ -580,6 +665,17 @@ void *claim_scratch(int &dev, int &code) {
   return g_scratch_ptr[dev];
 }
 
+// The batch's stream, the side stream of its captured forks and the fork
+// and join events; nonzero on failure.
+int make_streams(fd3d_batch *b) {
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
+    return 1;
+  return 0;
+}
+
 void release_scratch(int dev) {
   std::lock_guard<std::mutex> lock(g_scratch_mu);
   if (dev >= 0 && dev < kMaxDevices) g_scratch_busy[dev] = false;
@@ -604,7 +700,7 @@ int fd3d_batch_create(int max_problems, int tz, int mode, fd3d_batch **out) {
   b->tz = tz;
   b->mode = mode;
   auto bail = [&](int code) { fd3d_batch_destroy(b); return code; };
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(FD3D_E_HIP);
+  if (make_streams(b)) return bail(FD3D_E_HIP);
   if (hipMalloc(&b->d_dom, (size_t)max_problems * sizeof(BatchDomain)) != hipSuccess) return bail(FD3D_E_NOMEM);
   const size_t ncnt = kCntBase + (size_t)max_problems * K_COUNT;
   if (hipMalloc(&b->d_cnt, ncnt * sizeof(unsigned)) != hipSuccess) return bail(FD3D_E_NOMEM);
@@ -633,7 +729,7 @@ int fd3d_batch_create_static(int max_problems, int tz, fd3d_batch **out) {
   b->d_cnt = reinterpret_cast<unsigned *>(c + kStaticDomBytes + kStaticSegBytes);
   b->seg_cap = K_COUNT * kStaticMaxProblems;
   auto bail = [&](int rc) { fd3d_batch_destroy(b); return rc; };
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(FD3D_E_HIP);
+  if (make_streams(b)) return bail(FD3D_E_HIP);
   // the last batch's counters are still in the array: clear them
   const int ncnt = kCntBase + K_COUNT * kStaticMaxProblems;
   hipLaunchKernelGGL(k_zero_u32, dim3(cdiv(ncnt, 256)), dim3(256), 0, b->stream, b->d_cnt, ncnt);
@@ -678,7 +774,13 @@ int fd3d_batch_step(fd3d_batch *b, int nsteps) {
   }
   hipEvent_t e0 = take_event(b), e1 = take_event(b);
   BCHK(b, hipEventRecord(e0, b->stream));
-  for (int s = 0; s < nsteps; ++s) BCHK(b, hipGraphLaunch(b->exec, b->stream));
+  if (b->mode == FD3D_BATCH_PERSISTENT) {
+    for (int s = 0; s < nsteps; ++s) BCHK(b, hipGraphLaunch(b->exec[0], b->stream));
+  } else {
+    BCHK(b, hipGraphLaunch(b->exec[0], b->stream));
+    for (int s = 1; s < nsteps; ++s) BCHK(b, hipGraphLaunch(b->exec[1], b->stream));
+    BCHK(b, hipGraphLaunch(b->exec[2], b->stream));
+  }
   BCHK(b, hipEventRecord(e1, b->stream));
   b->evs.push_back({e0, e1, nsteps});
   for (fd3d_handle *h : b->mem) h->steps += nsteps;
@@ -744,8 +846,7 @@ void fd3d_batch_destroy(fd3d_batch *b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (fd3d_handle *h : b->mem) h->batch = nullptr;
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  drop_graphs(b);
   for (auto &e : b->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto e : b->pool) (void)hipEventDestroy(e);
   if (b->fixed_dev >= 0) {
@@ -755,6 +856,9 @@ void fd3d_batch_destroy(fd3d_batch *b) {
     if (b->d_seg) (void)hipFree(b->d_seg);
     if (b->d_cnt) (void)hipFree(b->d_cnt);
   }
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->join) (void)hipEventDestroy(b->join);
+  if (b->side) (void)hipStreamDestroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
