@@ -244,7 +244,8 @@ class Server:
 
 
 def run_plain(requests: List[Request], keep_final: bool = True, clock=time.perf_counter,
-              log: Optional[Callable[[str], None]] = None) -> List[Result]:
+              log: Optional[Callable[[str], None]] = None,
+              solver_args: Optional[Callable[[Request], Dict]] = None) -> List[Result]:
     """Back-to-back plain solves in arrival order (one request at a time).
     Service times are measured; the queueing timeline of a FIFO single
     server then follows from the arrival times, so the run does not wait
@@ -253,20 +254,30 @@ def run_plain(requests: List[Request], keep_final: bool = True, clock=time.perf_
     free_at = 0.0
     for r in sorted(requests, key=lambda q: q.arrival):
         t1 = clock()
-        with capi.Solver(r.case) as s:
+        err = None
+        fin, bins = {}, {}
+        with capi.Solver(r.case, **(solver_args(r) if solver_args else {})) as s:
             if r.bins is not None:
                 s.set_source_bins(*r.bins, nbins=r.steps)
             autotune.apply(s)
             s.step(r.steps, outputs=False)
-            s.sync()
-            fin = s.final() if keep_final else {}
-            bins = s.source_bins() if r.bins is not None else {}
+            try:
+                s.sync()
+            except capi.FD3DError as e:
+                # as in Server: a Newton cap or non-finite outputs fail this
+                # request alone
+                if e.code not in (-6, -7):
+                    raise
+                err = str(e)
+            if err is None:
+                fin = s.final() if keep_final else {}
+                bins = s.source_bins() if r.bins is not None else {}
         service = clock() - t1
         start = max(free_at, r.arrival)
         free_at = start + service
-        results.append(Result(r.rid, r.arrival, start, free_at, r.steps, r.case.cells, fin, bins))
+        results.append(Result(r.rid, r.arrival, start, free_at, r.steps, r.case.cells, fin, bins, error=err))
         if log:
-            log(f"[plain] request {r.rid}: {service:.3f} s")
+            log(f"[plain] request {r.rid}: {service:.3f} s" + (f", failed ({err})" if err else ""))
     return results
 
 

@@ -91,3 +91,10 @@ def test_failed_member_leaves_the_others_running():
             assert np.array_equal(r.bins[k], v), f"request {rid} bins {k}"
     stats = serve.summary(list(got.values()), srv.makespan)
     assert stats["failed"] == 1 and stats["requests"] == len(reqs) - 1
+    # the back-to-back baseline fails the same request alone too
+    alone = {r.rid: r for r in serve.run_plain(reqs, solver_args=lambda r: {"newton_max": 1} if r.rid == bad else {})}
+    assert alone[bad].error is not None and "FD3D_E_NEWTON" in alone[bad].error, alone[bad].error
+    for rid, ref in plain.items():
+        assert alone[rid].error is None, (rid, alone[rid].error)
+        for k, v in ref.final.items():
+            assert np.array_equal(alone[rid].final[k], v), f"request {rid} final {k} (back to back)"

@@ -83,3 +83,51 @@ def test_stream_misuse_and_flags():
         with pytest.raises(capi.FD3DError) as e:
             s.step_wait()
         assert e.value.code == -6         # FD3D_E_NEWTON from the chunk's own flags
+
+
+# Calls of 1, 2, 3 and 4 steps, streamed chunks, steps without outputs and
+# kernel-by-kernel (profiled) steps, mixed: 40 steps in all.
+CALLS = [1, 2, ("stream", 5, 2), 3, ("prof", 2), ("quiet", 3), 1, ("stream", 8, 8), 4, ("prof", 1),
+         ("stream", 3, 1), 2, ("quiet", 1), 4]
+
+
+@pytest.mark.parametrize("friction,fspace", [("sw", True), ("fvw", False)], ids=["sw_fs", "fvw_free"])
+def test_call_pattern_equals_one_call(friction, fspace):
+    """A run of steps replays a head graph, rotated graphs (each step's
+    fault-plane kernels beside the next step's velocity sweeps) and a tail
+    graph, so every call returns with its last step complete.  Any mix of
+    call sizes and forms must then give, bit for bit, the slabs, stations,
+    final arrays and the nine 3D fields of one call of the same length."""
+    case = with_stations(C.tiny(friction, fspace, ntfd=40, seed=74))
+    assert sum(c if isinstance(c, int) else c[1] for c in CALLS) == case.ntfd
+    say(f"[calls] {case.name}: {len(CALLS)} calls, {case.ntfd} steps")
+    with capi.Solver(case, ring_steps=16) as s:
+        ref = s.step(case.ntfd)
+        ref_fin = s.final()
+        ref_f = {n: s.field(n) for n in capi.FIELDS}
+    got = {k: [] for k in KEYS}
+    with capi.Solver(case, ring_steps=16) as s:
+        for c in CALLS:
+            if isinstance(c, int):
+                parts = [s.step(c)]
+            elif c[0] == "stream":
+                parts = list(s.stream(c[1], c[2]))
+            elif c[0] == "prof":
+                s.set_profiling(True)
+                parts = [s.step(c[1])]
+                s.set_profiling(False)
+            else:
+                s.step(c[1], outputs=False)
+                parts = [s.read_outputs(c[1])]
+            for r in parts:
+                for k in KEYS:
+                    got[k].append(r[k])
+        assert s.steps_done() == case.ntfd
+        fin = s.final()
+        for n in capi.FIELDS:
+            assert np.array_equal(s.field(n), ref_f[n]), n
+    for k in KEYS:
+        assert np.array_equal(np.concatenate(got[k]), ref[k]), k
+    for n, v in ref_fin.items():
+        assert np.array_equal(fin[n], v), n
+    say(f"[calls] {case.name}: bit-identical to one call")
