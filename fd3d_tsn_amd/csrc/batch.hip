@@ -583,7 +583,12 @@ int rebuild(fd3d_batch *b) {
   }
   const int kinds = b->mode == FD3D_BATCH_PERSISTENT ? 1 : 3;
   for (int kind = 0; kind < kinds; ++kind) {
-    BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t eb = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (eb != hipSuccess) {
+      drop_graphs(b);   // all graphs or none
+      b->dirty = true;
+      return bfail(b, FD3D_E_HIP, std::string("begin capture: ") + hipGetErrorString(eb));
+    }
     const int key = b->tz * 2 + (b->fvw ? 1 : 0);
     if (b->mode == FD3D_BATCH_PERSISTENT) {
       switch (key) {

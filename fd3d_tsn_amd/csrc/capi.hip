@@ -752,7 +752,11 @@ static int capture_graphs(fd3d_handle *h) {
   while (nlead < pre.size() && (pre[nlead] == K_VEL_INTERIOR || pre[nlead] == K_VEL_FAULT || pre[nlead] == K_PML_VEL))
     ++nlead;
   for (int kind = 0; kind < 3; ++kind) {
-    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t eb = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (eb != hipSuccess) {
+      drop_graph(h);   // all three graphs or none
+      return fail(h, FD3D_E_HIP, std::string("begin capture: ") + hipGetErrorString(eb));
+    }
     hipError_t e = hipSuccess;
     std::string what;
     auto on = [&](KernelId id, hipStream_t st) {
@@ -817,7 +821,8 @@ static int run_steps(fd3d_handle *h, int n) {
       }
     return 0;
   }
-  if (!h->gexec[0]) {
+  if (!h->gexec[0] || !h->gexec[1] || !h->gexec[2]) {
+    drop_graph(h);
     int rc = capture_graphs(h);
     if (rc) return rc;
   }
