@@ -238,11 +238,11 @@ void fd3d_destroy(fd3d_handle *h);
  * (dynamicsolver.f90:66 calls fd3d() once); a dynamic inversion runs many
  * such forward models back to back.  A batch advances many resident
  * problems of different sizes, friction laws and free-surface settings
- * together: each time step is one launch in which workgroups take
- * (problem, stage, block) items from a device queue, a stage of a problem
- * starting when the stage before it in that problem has finished; the step
- * is captured once in a HIP graph and replayed.  Results are bit-identical
- * to fd3d_step on each problem alone.
+ * together (FD3D_BATCH_STAGED): each stage of a step is one launch covering
+ * every problem, whose workgroups take (problem, stage, block) items from a
+ * table; the launches of a step are captured in HIP graphs and replayed,
+ * with each step's fault-plane stages beside the next step's velocity
+ * stages.  Results are bit-identical to fd3d_step on each problem alone.
  * ------------------------------------------------------------------------ */
 typedef struct fd3d_batch fd3d_batch;
 
