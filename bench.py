@@ -51,7 +51,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="large_sw", choices=["large_sw", "plain_sw", "fvw", "batched", "served", "served-cpu"])
+    ap.add_argument("--workload", default="large_sw",
+                    choices=["large_sw", "plain_sw", "fvw", "batched", "served", "served-cpu", "cpu-reference"])
     ap.add_argument("--batch-mode", default="staged", choices=["persistent", "staged"])
     ap.add_argument("--batch-tz", type=int, default=16)
     ap.add_argument("--batch-fused", default="on", choices=["on", "off"],
@@ -152,6 +153,37 @@ def cpu_baseline(case, steps):
     t = time.perf_counter() - t0
     return {"value": sample.cells * steps / t / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
             "sample": f"{case.name}: {steps} steps of the C++ oracle restatement, serial ({t:.2f} s)"}
+
+
+def bench_cpu_reference(a):
+    """The reference CPU build alone on the 1024x512x512 grid for --cpu-steps
+    steps, in runs of at most 20 steps (each run prints its time), pinned to
+    core 0; the rate comes from the runs' own 'CPU TIME OF TIME LOOP'."""
+    from fd3d_tsn_amd import cases as C
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refrun
+    case = C.c2_large_sw()
+    model = "unknown"
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    left, loop, runs = a.cpu_steps, 0.0, []
+    while left > 0:
+        n = min(20, left)
+        s = case.replace(ntfd=n, stations=[])
+        _, wall, t = refrun.run(s, fast=True, keep_outputs=False, cores="0")
+        loop += t
+        runs.append({"steps": n, "loop_s": t, "wall_s": wall})
+        left -= n
+        print(f"[bench] cpu reference: {a.cpu_steps - left} of {a.cpu_steps} steps, this run {t:.2f} s",
+              file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "Gcell-updates/s (reference CPU build)", "value": case.cells * a.cpu_steps / loop / 1e9,
+                      "unit": "Gcell-updates/s", "cores": 1, "kind": "reference", "steps": a.cpu_steps,
+                      "ms_per_step": loop / a.cpu_steps * 1e3, "cpu_model": model,
+                      "config": {"workload": case.name, "grid": [case.nxt, case.nyt, case.nzt]},
+                      "note": "reference Fortran built with flang -O3 -ffast-math, serial, pinned to core 0; "
+                              "time = the runs' own 'CPU TIME OF TIME LOOP'", "runs": runs}))
 
 
 def bench_batched(a, emit=True):
@@ -397,6 +429,8 @@ def main():
         return bench_served(a)
     if a.workload == "served-cpu":
         return bench_served_cpu(a)
+    if a.workload == "cpu-reference":
+        return bench_cpu_reference(a)
     import numpy as np
     from fd3d_tsn_amd import capi
     from fd3d_tsn_amd import cases as C
