@@ -329,7 +329,8 @@ def bench_served(a):
            "config": {"workload": f"served_stream_first_{len(reqs)}_of_2000", "requests": len(reqs),
                       "rate_per_s": 20.0, "slots": a.serve_slots, "chunk_steps": a.serve_chunk,
                       "cell_updates": upd},
-           "served": dict(ss, gpu_step_s=srv.gpu_step_s, host_wait_s=srv.gpu_busy_s, chunks=srv.chunks) if ss else None,
+           "served": dict(ss, gpu_step_s=srv.gpu_step_s, host_wait_s=srv.gpu_busy_s, chunks=srv.chunks,
+                          host_s=srv.host_s) if ss else None,
            "plain_back_to_back": ps,
            "speedup_vs_plain": ss["ruptures_per_s"] / ps["ruptures_per_s"] if ss and ps else None,
            "max_rel_l2_vs_plain": l2}

@@ -148,6 +148,10 @@ class Server:
         self.gpu_busy_s = 0.0      # host time spent waiting for chunks
         self.gpu_step_s = 0.0      # GPU time of the batch steps (HIP events)
         self.chunks = 0
+        # host time by phase: queueing a chunk (with the batch's graph
+        # re-capture after a membership change), preparing admissions,
+        # retiring finished members
+        self.host_s = {"submit": 0.0, "admit": 0.0, "retire": 0.0}
 
     def close(self):
         self.batch.close()
@@ -172,8 +176,11 @@ class Server:
                 active.append(a)
             ready = []
             n = min([self.chunk] + [a["left"] for a in active]) if active else 0
+            t1 = clock()
             if n:
                 self.batch.step(n)          # asynchronous: the host prepares admissions meanwhile
+            t2 = clock()
+            self.host_s["submit"] += t2 - t1
             # admission: a free slot and the footprint in free HBM
             while queue and len(active) + len(ready) < self.max_slots and self._fits(queue[0], 0):
                 r = queue.popleft()
@@ -186,6 +193,7 @@ class Server:
                 if t is not None:
                     s.set_tuning(*t)
                 ready.append({"req": r, "solver": s, "left": r.steps, "admitted": clock() - t0})
+            self.host_s["admit"] += clock() - t2
             if queue and not active and not ready:
                 raise RuntimeError(f"request {queue[0].rid} does not fit in device memory")
             if n:
@@ -194,6 +202,7 @@ class Server:
                 self.gpu_busy_s += clock() - t1
                 self.gpu_step_s += self.batch.elapsed()[0] * 1e-3
                 self.chunks += 1
+                t3 = clock()
                 done = []
                 for a in active:
                     a["left"] -= n
@@ -213,6 +222,7 @@ class Server:
                     active.remove(a)
                     what = f"failed ({err})" if err else f"done: latency {results[-1].latency:.3f} s"
                     self.log(f"[serve] request {r.rid} {what}, {len(active)} active, {len(queue)} queued")
+                self.host_s["retire"] += clock() - t3
             elif not ready and pending:
                 time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
         self.makespan = clock() - t0
