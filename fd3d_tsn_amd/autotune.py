@@ -312,10 +312,11 @@ def batch_plan(cases: Iterable[C.Case], fused: bool = True) -> Tuple[int, List[O
 
 
 def main():
-    """python -m fd3d_tsn_amd.autotune [batched|served N|single|large] [N strips]: tune
-    the size classes of a workload on the GPU and update the cache (with
-    "strips", only the fused sweep's strip widths of the classes that take
-    it)."""
+    """python -m fd3d_tsn_amd.autotune [batched|served N|single|large] [N] [strips] [new]:
+    tune the size classes of a workload on the GPU and update the cache
+    (with "strips", only the fused sweep's strip widths of the classes that
+    take it; with "new", only classes the cache does not hold yet, or
+    whose strip widths it does not hold)."""
     import sys
     import time
     which = sys.argv[1] if len(sys.argv) > 1 else "batched"
@@ -329,13 +330,18 @@ def main():
         cs = [C.c1_plain_sw(), C.c2_large_sw(), C.c3_fvw()]
     else:
         cs = [C.c2_large_sw()]
-    strips_only = len(sys.argv) > 3 and sys.argv[3] == "strips"
+    flags = set(sys.argv[3:])
+    strips_only = "strips" in flags
+    only_new = "new" in flags
     seen = set()
     for c in cs:
         k = size_class(c)
         if k in seen:
             continue
         seen.add(k)
+        e = lookup(c)
+        if only_new and e is not None and (not strips_only or "fused_strip" in e):
+            continue
         t0 = time.perf_counter()
         if strips_only:
             # the fused sweep's strip widths only, for classes already in
