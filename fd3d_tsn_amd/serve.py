@@ -12,8 +12,11 @@ arrive as a stream and share one GPU:
   launch per stage covering every problem, the step replayed from a HIP
   graph), in chunks of steps no longer than the shortest remaining run;
 * while a chunk runs on the GPU, the host prepares the next requests
-  (inputs, device allocation, upload, prologue) on their own streams; they
-  join at the next chunk boundary, when finished problems leave;
+  (inputs, device allocation, upload, prologue) on their own streams, for
+  every slot that is free or is freed by a problem finishing with that
+  chunk; they join at the next chunk boundary, when finished problems
+  leave; the results of the problems that left are read, and their
+  memory freed, while the following chunk runs;
 * each result (the final 2D arrays of fd3d(), fd3d_theo.f90:604-658, and
   on request the inversion's patch slip rates and moment rate,
   fd3d_theo.f90:560-578) is bit-identical to the same request solved alone.
@@ -150,7 +153,8 @@ class Server:
         self.chunks = 0
         # host time by phase: queueing a chunk (with the batch's graph
         # re-capture after a membership change), preparing admissions,
-        # retiring finished members
+        # reading the results of finished members and freeing them (while
+        # the next chunk runs; freeing device memory may wait for it)
         self.host_s = {"submit": 0.0, "admit": 0.0, "retire": 0.0}
 
     def close(self):
@@ -165,9 +169,10 @@ class Server:
         queue: collections.deque = collections.deque()
         active: List[dict] = []
         ready: List[dict] = []          # prepared during the last chunk, join at the next boundary
+        leaving: List[dict] = []        # left the batch at the last boundary, read during the next chunk
         results: List[Result] = []
         t0 = clock()
-        while pending or queue or active or ready:
+        while pending or queue or active or ready or leaving:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
@@ -181,8 +186,10 @@ class Server:
                 self.batch.step(n)          # asynchronous: the host prepares admissions meanwhile
             t2 = clock()
             self.host_s["submit"] += t2 - t1
-            # admission: a free slot and the footprint in free HBM
-            while queue and len(active) + len(ready) < self.max_slots and self._fits(queue[0], 0):
+            # admission: a free slot (members whose run ends with this chunk
+            # leave at its end) and the footprint in free HBM
+            staying = sum(1 for a in active if a["left"] > n)
+            while queue and staying + len(ready) < self.max_slots and self._fits(queue[0], 0):
                 r = queue.popleft()
                 s = capi.Solver(r.case, **self.solver_args(r))
                 if r.bins is not None:
@@ -194,39 +201,44 @@ class Server:
                     s.set_tuning(*t)
                 ready.append({"req": r, "solver": s, "left": r.steps, "admitted": clock() - t0})
             self.host_s["admit"] += clock() - t2
-            if queue and not active and not ready:
+            if queue and not active and not ready and not leaving:
                 raise RuntimeError(f"request {queue[0].rid} does not fit in device memory")
+            # results of the members that left at the last boundary (they
+            # are out of the batch, so the chunk in flight does not touch them)
+            t3 = clock()
+            for a in leaving:
+                results.append(self._finish(a, lambda: clock() - t0))
+                what = f"failed ({a['error']})" if a["error"] else f"done: latency {results[-1].latency:.3f} s"
+                self.log(f"[serve] request {a['req'].rid} {what}, {len(active)} active, {len(queue)} queued")
+            leaving = []
+            self.host_s["retire"] += clock() - t3
             if n:
                 t1 = clock()
                 failed = self._sync()
                 self.gpu_busy_s += clock() - t1
                 self.gpu_step_s += self.batch.elapsed()[0] * 1e-3
                 self.chunks += 1
-                t3 = clock()
-                done = []
-                for a in active:
+                for a in list(active):
                     a["left"] -= n
                     if a["left"] == 0 or id(a["solver"]) in failed:
-                        done.append(a)
-                for a in done:
-                    s = a["solver"]
-                    r = a["req"]
-                    self.batch.remove(s)
-                    err = failed.get(id(s))
-                    fin = s.final() if self.keep_final and not err else {}
-                    bins = s.source_bins() if r.bins is not None and not err else {}
-                    outs = s.read_outputs(min(r.steps, s.ring_steps)) if self.keep_outputs and not err else {}
-                    s.close()
-                    results.append(Result(r.rid, r.arrival, a["admitted"], clock() - t0, r.steps, r.case.cells,
-                                          fin, bins, outs, err))
-                    active.remove(a)
-                    what = f"failed ({err})" if err else f"done: latency {results[-1].latency:.3f} s"
-                    self.log(f"[serve] request {r.rid} {what}, {len(active)} active, {len(queue)} queued")
-                self.host_s["retire"] += clock() - t3
+                        self.batch.remove(a["solver"])
+                        a["error"] = failed.get(id(a["solver"]))
+                        active.remove(a)
+                        leaving.append(a)
             elif not ready and pending:
                 time.sleep(max(0.0, pending[0].arrival - (clock() - t0)))
         self.makespan = clock() - t0
         return results
+
+    def _finish(self, a: dict, now: Callable[[], float]) -> Result:
+        """Read the result of a member that has left the batch and free it;
+        the request is finished once its arrays are on the host."""
+        s, r, err = a["solver"], a["req"], a["error"]
+        fin = s.final() if self.keep_final and not err else {}
+        bins = s.source_bins() if r.bins is not None and not err else {}
+        outs = s.read_outputs(min(r.steps, s.ring_steps)) if self.keep_outputs and not err else {}
+        s.close()
+        return Result(r.rid, r.arrival, a["admitted"], now(), r.steps, r.case.cells, fin, bins, outs, err)
 
     def _sync(self) -> Dict[int, str]:
         """Wait for the chunk.  A member whose rate-and-state Newton solve hit

@@ -44,3 +44,131 @@ def test_c5_stream_mix():
     for r in reqs[:50]:
         assert 64 <= r.case.nxtT <= 384 and 64 <= r.case.nytT <= 192 and 64 <= r.case.nztT <= 192
         assert 800 <= r.steps <= 3000 and r.case.cfl() <= 0.25
+
+
+class _FakeError(Exception):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class _FakeSolver:
+    """Stands in for capi.Solver: counts the steps the batch gives it."""
+    ring_steps = 64
+
+    def __init__(self, log, case, **kw):
+        self.log, self.case, self.done, self.closed, self.batch = log, case, 0, False, None
+        self.bad = kw.get("newton_max") == 1
+
+    def set_source_bins(self, *a, **k):
+        pass
+
+    def set_tuning(self, *a):
+        pass
+
+    def sync(self):
+        if self.bad:
+            raise _FakeError(-6, "FD3D_E_NEWTON: capped")
+
+    def final(self):
+        assert self.batch is None and not self.closed, "results read while in the batch or after close"
+        return {"steps": np.array([self.done])}
+
+    def close(self):
+        assert self.batch is None and not self.closed
+        self.closed = True
+        self.log.append(("close", self.case.rid))
+
+
+class _FakeBatch:
+    """Stands in for capi.Batch: records each chunk's members and advances them."""
+
+    def __init__(self, log, max_slots, **kw):
+        self.log, self.max_slots, self.members, self.in_flight = log, max_slots, [], False
+
+    def add(self, s):
+        assert not self.in_flight and len(self.members) < self.max_slots and s.batch is None
+        s.batch = self
+        self.members.append(s)
+
+    def remove(self, s):
+        assert not self.in_flight
+        s.batch = None
+        self.members.remove(s)
+
+    def step(self, n):
+        assert not self.in_flight and self.members
+        for s in self.members:
+            assert s.done + n <= s.case.ntfd, "a member stepped past its run"
+            s.done += n
+        self.in_flight = True
+        self.log.append(("chunk", n, sorted(s.case.rid for s in self.members)))
+
+    def sync(self):
+        self.in_flight = False
+        if any(s.bad for s in self.members):
+            raise _FakeError(-6, "FD3D_E_NEWTON in a member")
+
+    def elapsed(self):
+        return (1.0,)
+
+    def close(self):
+        pass
+
+
+class _Case:
+    def __init__(self, rid, ntfd):
+        self.rid, self.ntfd, self.cells = rid, ntfd, 1000
+
+
+def _fake_server(monkeypatch, log, **kw):
+    class FakeCapi:
+        STAGED = 1
+        FD3DError = _FakeError
+        Batch = staticmethod(lambda max_slots, **k: _FakeBatch(log, max_slots, **k))
+        Solver = staticmethod(lambda case, **k: _FakeSolver(log, case, **k))
+        device_memory = staticmethod(lambda: (1 << 40, 1 << 40))
+        footprint = staticmethod(lambda case: 1 << 20)
+
+    monkeypatch.setattr(serve, "capi", FakeCapi)
+    monkeypatch.setattr(serve.autotune, "choose", lambda case, tz: None)
+    monkeypatch.setattr(serve.autotune, "batch_member", lambda *a: None)
+    return serve.Server(mode=FakeCapi.STAGED, reserve_bytes=0, **kw)
+
+
+def test_server_refills_slots_at_the_boundary_they_free(monkeypatch):
+    """Scheduling of the serving loop, on fakes: every request gets exactly
+    its steps; no chunk runs with a free slot while requests wait (a slot
+    freed at a chunk's end is filled at that boundary); results are read
+    only after the member left the batch, and every solver is freed once."""
+    log = []
+    steps = [20, 8, 35, 13, 8, 50, 27, 16, 9, 40]
+    reqs = [serve.Request(q, 0.0, _Case(q, n)) for q, n in enumerate(steps)]
+    srv = _fake_server(monkeypatch, log, max_slots=3, chunk=8)
+    got = srv.run(reqs)
+    assert sorted(r.rid for r in got) == list(range(len(reqs)))
+    for r in got:
+        assert r.error is None and int(r.final["steps"][0]) == steps[r.rid]
+    assert sorted(e[1] for e in log if e[0] == "close") == list(range(len(reqs)))
+    chunks = [e for e in log if e[0] == "chunk"]
+    started = set()
+    for _tag, n, members in chunks:
+        started.update(members)
+        waiting = len(reqs) - len(started)
+        assert len(members) == 3 or waiting == 0, (members, waiting)
+    assert srv.chunks == len(chunks)
+
+
+def test_server_failed_member_leaves_alone(monkeypatch):
+    """A member whose Newton solve hits its cap leaves with the error after
+    the chunk that reported it; the others run to the end."""
+    log = []
+    steps = [24, 24, 24, 24]
+    reqs = [serve.Request(q, 0.0, _Case(q, n)) for q, n in enumerate(steps)]
+    srv = _fake_server(monkeypatch, log, max_slots=3, chunk=8,
+                       solver_args=lambda r: {"newton_max": 1} if r.rid == 1 else {})
+    got = {r.rid: r for r in srv.run(reqs)}
+    assert got[1].error is not None and "FD3D_E_NEWTON" in got[1].error and got[1].final == {}
+    for q in (0, 2, 3):
+        assert got[q].error is None and int(got[q].final["steps"][0]) == 24
+    assert sorted(e[1] for e in log if e[0] == "close") == [0, 1, 2, 3]
