@@ -289,6 +289,19 @@ __global__ void k_zero_u32(unsigned *p, int n) {
   if (i < n) p[i] = 0u;
 }
 
+// The batch's report after a chunk, gathered so that the host reads it in
+// one copy: rep[0] the sticky failed-wait count, then per member q its
+// control block's Newton-cap count and non-finite flag.
+__global__ void k_batch_report(const BatchDomain *dom, int n, const unsigned *cnt, unsigned *rep) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q == 0) rep[0] = cnt[1];
+  if (q < n) {
+    const Ctl *c = dom[q].p.ctl;
+    rep[1 + 2 * q] = (unsigned)c->newton_capped;
+    rep[2 + 2 * q] = (unsigned)c->nonfinite;
+  }
+}
+
 }  // namespace
 
 // Fixed device tables of the batch that allocates nothing
@@ -301,7 +314,8 @@ constexpr int kStaticMaxProblems = 64;
 constexpr size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 constexpr size_t kStaticDomBytes = align256(sizeof(BatchDomain) * kStaticMaxProblems);
 constexpr size_t kStaticSegBytes = align256(sizeof(BatchSeg) * K_COUNT * kStaticMaxProblems);
-constexpr size_t kStaticCntBytes = align256(sizeof(unsigned) * (kCntBase + K_COUNT * kStaticMaxProblems));
+constexpr size_t kStaticCntBytes =
+    align256(sizeof(unsigned) * (kCntBase + K_COUNT * kStaticMaxProblems + 1 + 2 * kStaticMaxProblems));
 constexpr size_t kScratchBytes = kStaticDomBytes + kStaticSegBytes + kStaticCntBytes;
 static_assert(kScratchBytes <= (1u << 20), "the fixed batch tables stay under 1 MiB per GPU");
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -315,6 +329,8 @@ struct fd3d_batch {
   BatchDomain *d_dom = nullptr;
   BatchSeg *d_seg = nullptr;
   unsigned *d_cnt = nullptr;
+  unsigned *d_rep = nullptr;    // report after a chunk (k_batch_report), behind the counters in d_cnt
+  std::vector<unsigned> rep;    // its host copy: 1 + 2 * maxdom words
   int seg_cap = 0;
   std::vector<BatchSeg> segs;
   int stage_seg0[K_COUNT + 1] = {0};
@@ -554,8 +570,10 @@ int rebuild(fd3d_batch *b) {
   b->stage_seg0[K_COUNT] = (int)b->segs.size();
   b->stage_item0[K_COUNT] = total;
   b->total = total;
-  b->dirty = false;
-  if (b->mem.empty()) return 0;
+  if (b->mem.empty()) {
+    b->dirty = false;
+    return 0;
+  }
   if ((int)b->segs.size() > b->seg_cap) {
     if (b->fixed_dev >= 0)   // cannot happen: one segment per (member, stage) fits the fixed table
       return bfail(b, FD3D_E_INVALID, "batch: segment table of the fixed scratch exceeded");
@@ -568,6 +586,7 @@ int rebuild(fd3d_batch *b) {
   BCHK(b, hipMemcpyAsync(b->d_seg, b->segs.data(), b->segs.size() * sizeof(BatchSeg), hipMemcpyHostToDevice,
                          b->stream));
   BCHK(b, hipStreamSynchronize(b->stream));
+  b->dirty = false;   // the device descriptors are the members' (fd3d_batch_sync relies on it)
   int rc = 0;
   b->workers = 0;
   if (b->mode == FD3D_BATCH_PERSISTENT) {
@@ -708,7 +727,10 @@ int fd3d_batch_create(int max_problems, int tz, int mode, fd3d_batch **out) {
   if (make_streams(b)) return bail(FD3D_E_HIP);
   if (hipMalloc(&b->d_dom, (size_t)max_problems * sizeof(BatchDomain)) != hipSuccess) return bail(FD3D_E_NOMEM);
   const size_t ncnt = kCntBase + (size_t)max_problems * K_COUNT;
-  if (hipMalloc(&b->d_cnt, ncnt * sizeof(unsigned)) != hipSuccess) return bail(FD3D_E_NOMEM);
+  if (hipMalloc(&b->d_cnt, (ncnt + 1 + 2 * (size_t)max_problems) * sizeof(unsigned)) != hipSuccess)
+    return bail(FD3D_E_NOMEM);
+  b->d_rep = b->d_cnt + ncnt;
+  b->rep.assign(1 + 2 * (size_t)max_problems, 0u);
   if (hipMemsetAsync(b->d_cnt, 0, ncnt * sizeof(unsigned), b->stream) != hipSuccess) return bail(FD3D_E_HIP);
   if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(FD3D_E_HIP);
   *out = b;
@@ -732,6 +754,8 @@ int fd3d_batch_create_static(int max_problems, int tz, fd3d_batch **out) {
   b->d_dom = reinterpret_cast<BatchDomain *>(c);
   b->d_seg = reinterpret_cast<BatchSeg *>(c + kStaticDomBytes);
   b->d_cnt = reinterpret_cast<unsigned *>(c + kStaticDomBytes + kStaticSegBytes);
+  b->d_rep = b->d_cnt + kCntBase + K_COUNT * kStaticMaxProblems;
+  b->rep.assign(1 + 2 * (size_t)max_problems, 0u);
   b->seg_cap = K_COUNT * kStaticMaxProblems;
   auto bail = [&](int rc) { fd3d_batch_destroy(b); return rc; };
   if (make_streams(b)) return bail(FD3D_E_HIP);
@@ -794,13 +818,35 @@ int fd3d_batch_step(fd3d_batch *b, int nsteps) {
 
 int fd3d_batch_sync(fd3d_batch *b) {
   if (!b) return FD3D_E_INVALID;
+  const int n = (int)b->mem.size();
+  if (b->dirty) {
+    // membership changed since the last step: the device descriptors are
+    // not the members', so read each member's control block
+    BCHK(b, hipStreamSynchronize(b->stream));
+    unsigned flag = 0;
+    BCHK(b, hipMemcpyAsync(&flag, b->d_cnt + 1, sizeof(unsigned), hipMemcpyDeviceToHost, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    if (flag) return bfail(b, FD3D_E_HIP, "batch: a stage dependency wait ran out of time");
+    for (int q = 0; q < n; ++q) {
+      fd3d_handle *h = b->mem[q];
+      int rc = fd3d_handle_check_ctl(h, b->stream);
+      if (rc) return bfail(b, rc, "member " + std::to_string(q) + ": " + h->err);
+    }
+    return 0;
+  }
+  // one gather kernel and one copy after the queued steps, instead of a
+  // round trip per member
+  hipLaunchKernelGGL(k_batch_report, dim3(cdiv(std::max(n, 1), 64)), dim3(64), 0, b->stream, b->d_dom, n, b->d_cnt,
+                     b->d_rep);
+  BCHK(b, hipGetLastError());
+  BCHK(b, hipMemcpyAsync(b->rep.data(), b->d_rep, (1 + 2 * (size_t)n) * sizeof(unsigned), hipMemcpyDeviceToHost,
+                         b->stream));
   BCHK(b, hipStreamSynchronize(b->stream));
-  unsigned flag = 0;
-  BCHK(b, hipMemcpy(&flag, b->d_cnt + 1, sizeof(unsigned), hipMemcpyDeviceToHost));
-  if (flag) return bfail(b, FD3D_E_HIP, "batch: a stage dependency wait ran out of time");
-  for (size_t q = 0; q < b->mem.size(); ++q) {
+  if (b->rep[0]) return bfail(b, FD3D_E_HIP, "batch: a stage dependency wait ran out of time");
+  for (int q = 0; q < n; ++q) {
+    if (b->rep[1 + 2 * q] == 0u && b->rep[2 + 2 * q] == 0u) continue;
     fd3d_handle *h = b->mem[q];
-    int rc = fd3d_handle_check_ctl(h, b->stream);
+    int rc = fd3d_handle_check_ctl(h, b->stream);   // the member's error code and message
     if (rc) return bfail(b, rc, "member " + std::to_string(q) + ": " + h->err);
   }
   return 0;

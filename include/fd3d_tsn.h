@@ -262,7 +262,7 @@ int fd3d_batch_create(int max_problems, int tz, int mode, fd3d_batch **out);
  * code under HIP graph capture, harnesses that forbid allocation): the same
  * batch as fd3d_batch_create(max_problems, tz, FD3D_BATCH_STAGED, out),
  * whose member descriptors, segment table and counters live in a fixed
- * device array of the library (one per GPU, 125 KiB, held until the
+ * device array of the library (one per GPU, 125.5 KiB, held until the
  * process ends) instead of device allocations.  Creating, adding, removing,
  * stepping and destroying such a batch call no hipMalloc or hipFree; the
  * members' own memory comes from fd3d_create as before.  At most 64
