@@ -34,18 +34,22 @@ struct fd3d_handle {
   double kms[K_COUNT] = {0};
   long kcount[K_COUNT] = {0};
   // step sequence, and three graphs of it (replayed by fd3d_step when
-  // profiling is off; rebuilt when the sequence changes): a run of n steps
-  // is the head graph (a step up to advance), n-1 rotated graphs (the
-  // previous step's fault-plane kernels beside this step's velocity
-  // sweeps, then this step up to advance) and the tail graph (the last
-  // step's fault-plane kernels)
+  // profiling is off; captured after the first run of a sequence, which is
+  // queued directly): a run of n steps is the head part (a step up to
+  // advance), n-1 rotated parts (the previous step's fault-plane kernels
+  // beside this step's velocity sweeps, then this step up to advance) and
+  // the tail part (the last step's fault-plane kernels)
   std::vector<KernelId> seq;
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
-  // second stream and fork/join events of the captured step: the kernels of
-  // a half-step that write disjoint cells run side by side
+  // second stream and fork/join events of a step: the kernels of a
+  // half-step that write disjoint cells run side by side
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  // streams and events the graphs are captured on (nothing runs on them)
+  hipStream_t cap = nullptr, cap_side = nullptr;
+  hipEvent_t cap_fork = nullptr, cap_join = nullptr;
+  bool capture_failed = false;   // the last capture failed: queue steps directly
   // the batch this problem is advanced in (fd3d_batch_add), or null
   fd3d_batch *batch = nullptr;
   // overlapped output stream (fd3d_step_begin / fd3d_step_wait): two pinned
