@@ -65,12 +65,6 @@ struct BatchSeg {
   int start, dom, stage, pad;
 };
 
-// The two streams and the fork/join events a staged step is queued on.
-struct Lanes {
-  hipStream_t main, side;
-  hipEvent_t fork, join;
-};
-
 struct BatchArgs {
   const BatchSeg *seg;
   const BatchDomain *dom;
@@ -353,11 +347,7 @@ struct fd3d_batch {
   hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
   hipStream_t side = nullptr;               // the fork's second stream
   hipEvent_t fork = nullptr, join = nullptr;
-  // streams and events the staged graphs are captured on (nothing runs on
-  // them); capture_failed: the last capture failed, queue steps directly
-  hipStream_t cap = nullptr, cap_side = nullptr;
-  hipEvent_t cap_fork = nullptr, cap_join = nullptr;
-  bool capture_failed = false;
+  bool capture_failed = false;   // the last capture failed: queue steps directly
   struct Ev { hipEvent_t a, b; int n; };
   std::vector<Ev> evs;
   std::vector<hipEvent_t> pool;
@@ -660,14 +650,19 @@ int rebuild(fd3d_batch *b) {
 }
 
 // Staged mode: the head, rotated and tail graphs, captured on the capture
-// streams (which run nothing), so that the host captures them while the
-// GPU runs the steps just queued directly.  All three graphs or none; on
+// lanes (which run nothing), so that the host captures them while the GPU
+// runs the steps just queued directly.  All three graphs or none; on
 // failure the batch keeps queueing its steps directly until the next
 // membership change.
 int capture_staged(fd3d_batch *b) {
-  const Lanes L{b->cap, b->cap_side, b->cap_fork, b->cap_join};
+  Lanes L{};
+  std::unique_lock<std::mutex> lock;
+  if (fd3d_capture_lanes(L, lock) != 0) {
+    b->capture_failed = true;
+    return bfail(b, FD3D_E_HIP, "capture streams could not be made");
+  }
   for (int kind = 0; kind < 3; ++kind) {
-    const hipError_t eb = hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal);
+    const hipError_t eb = hipStreamBeginCapture(L.main, hipStreamCaptureModeThreadLocal);
     if (eb != hipSuccess) {
       drop_graphs(b);
       b->capture_failed = true;
@@ -675,7 +670,7 @@ int capture_staged(fd3d_batch *b) {
     }
     const int rc = queue_step(b, kind, L);
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(b->cap, &g);
+    hipError_t e = hipStreamEndCapture(L.main, &g);
     if (rc || e != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
       drop_graphs(b);
@@ -741,11 +736,7 @@ int make_streams(fd3d_batch *b) {
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&b->cap_side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&b->cap_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->cap_join, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
     return 1;
   return 0;
 }
@@ -970,10 +961,6 @@ void fd3d_batch_destroy(fd3d_batch *b) {
   }
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->join) (void)hipEventDestroy(b->join);
-  if (b->cap_fork) (void)hipEventDestroy(b->cap_fork);
-  if (b->cap_join) (void)hipEventDestroy(b->cap_join);
-  if (b->cap_side) (void)hipStreamDestroy(b->cap_side);
-  if (b->cap) (void)hipStreamDestroy(b->cap);
   if (b->side) (void)hipStreamDestroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;

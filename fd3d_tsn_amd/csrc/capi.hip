@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -404,6 +405,36 @@ int fd3d_handle_check_ctl(fd3d_handle *h, hipStream_t s) {
   return 0;
 }
 
+// Capture lanes per device (declared in handle.h).
+namespace {
+constexpr int kCaptureDevices = 64;
+std::mutex g_capture_mu;
+Lanes g_capture[kCaptureDevices];
+}  // namespace
+
+int fd3d_capture_lanes(Lanes &out, std::unique_lock<std::mutex> &lock) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kCaptureDevices) return FD3D_E_HIP;
+  lock = std::unique_lock<std::mutex>(g_capture_mu);
+  Lanes &L = g_capture[dev];
+  if (!L.main) {
+    Lanes n{};
+    if (hipStreamCreateWithFlags(&n.main, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&n.side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&n.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&n.join, hipEventDisableTiming) != hipSuccess) {
+      if (n.fork) (void)hipEventDestroy(n.fork);
+      if (n.side) (void)hipStreamDestroy(n.side);
+      if (n.main) (void)hipStreamDestroy(n.main);
+      lock.unlock();
+      return FD3D_E_HIP;
+    }
+    L = n;
+  }
+  out = L;
+  return 0;
+}
+
 extern "C" {
 
 size_t fd3d_footprint(const fd3d_config *cfg) {
@@ -453,11 +484,7 @@ int fd3d_create(const fd3d_config *cfg, fd3d_handle **out) {
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(FD3D_E_HIP);
   if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->join, hipEventDisableTiming) != hipSuccess ||
-      hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&h->cap_side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&h->cap_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->cap_join, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&h->join, hipEventDisableTiming) != hipSuccess)
     return bail(FD3D_E_HIP);
   const size_t n3 = (size_t)p.g.n3, n2 = (size_t)cfg->nxt * cfg->nzt;
   // two wavefield sets (the step reads one and writes the other)
@@ -747,11 +774,6 @@ constexpr long long kOverlapCells = 4LL << 20;
 // of kernels of the same half-step group (half_step_group) the first goes
 // on the main stream and the rest on the side stream (fork and join by
 // events), so they run side by side.
-struct Lanes {
-  hipStream_t main, side;
-  hipEvent_t fork, join;
-};
-
 // Queues one part on the lanes L (launches, or a capture when L.main is
 // capturing); what names the failing launch.
 static hipError_t queue_part(fd3d_handle *h, int kind, const Lanes &L, std::string &what) {
@@ -799,13 +821,15 @@ static hipError_t queue_part(fd3d_handle *h, int kind, const Lanes &L, std::stri
   return e;
 }
 
-// The three parts captured into graphs on the capture streams, which run
+// The three parts captured into graphs on the capture lanes, which run
 // nothing, so that the host captures them while the GPU runs the steps
 // just queued directly.  All three graphs or none.
 static int capture_graphs(fd3d_handle *h) {
-  const Lanes L{h->cap, h->cap_side, h->cap_fork, h->cap_join};
+  Lanes L{};
+  std::unique_lock<std::mutex> lock;
+  if (fd3d_capture_lanes(L, lock) != 0) return fail(h, FD3D_E_HIP, "capture streams could not be made");
   for (int kind = 0; kind < 3; ++kind) {
-    const hipError_t eb = hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal);
+    const hipError_t eb = hipStreamBeginCapture(L.main, hipStreamCaptureModeThreadLocal);
     if (eb != hipSuccess) {
       drop_graph(h);
       return fail(h, FD3D_E_HIP, std::string("begin capture: ") + hipGetErrorString(eb));
@@ -813,7 +837,7 @@ static int capture_graphs(fd3d_handle *h) {
     std::string what;
     const hipError_t e = queue_part(h, kind, L, what);
     hipGraph_t g = nullptr;
-    const hipError_t ee = hipStreamEndCapture(h->cap, &g);
+    const hipError_t ee = hipStreamEndCapture(L.main, &g);
     if (e != hipSuccess || ee != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
       drop_graph(h);
@@ -1187,10 +1211,6 @@ void fd3d_destroy(fd3d_handle *h) {
   if (h->fork) (void)hipEventDestroy(h->fork);
   if (h->join) (void)hipEventDestroy(h->join);
   if (h->side) (void)hipStreamDestroy(h->side);
-  if (h->cap_fork) (void)hipEventDestroy(h->cap_fork);
-  if (h->cap_join) (void)hipEventDestroy(h->cap_join);
-  if (h->cap_side) (void)hipStreamDestroy(h->cap_side);
-  if (h->cap) (void)hipStreamDestroy(h->cap);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -10,6 +11,18 @@
 #include "fd3d_dev.h"
 
 using namespace fd3d;
+
+// The two streams and the fork/join events a step is queued on.
+struct Lanes {
+  hipStream_t main, side;
+  hipEvent_t fork, join;
+};
+
+// The lanes step graphs are captured on: one set per device, shared by every
+// problem and batch of the process, since nothing runs on them.  Made at
+// first use on the current device; a capture holds the returned lock.
+// Returns 0 or FD3D_E_HIP (capi.hip).
+int fd3d_capture_lanes(Lanes &out, std::unique_lock<std::mutex> &lock);
 
 struct fd3d_handle {
   fd3d_config cfg{};
@@ -46,9 +59,6 @@ struct fd3d_handle {
   // half-step that write disjoint cells run side by side
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
-  // streams and events the graphs are captured on (nothing runs on them)
-  hipStream_t cap = nullptr, cap_side = nullptr;
-  hipEvent_t cap_fork = nullptr, cap_join = nullptr;
   bool capture_failed = false;   // the last capture failed: queue steps directly
   // the batch this problem is advanced in (fd3d_batch_add), or null
   fd3d_batch *batch = nullptr;

@@ -134,8 +134,13 @@ class Server:
     def __init__(self, max_slots: int = 32, chunk: int = 16, tz: int = 16, mode: int = capi.STAGED,
                  reserve_bytes: int = 4 << 30, keep_final: bool = True,
                  log: Optional[Callable[[str], None]] = None, fused: bool = True,
-                 keep_outputs: bool = False, solver_args: Optional[Callable[[Request], Dict]] = None):
+                 keep_outputs: bool = False, solver_args: Optional[Callable[[Request], Dict]] = None,
+                 prefetch: int = 2):
         self.max_slots = max_slots
+        # requests prepared ahead of a free slot: a slot freed after a short
+        # chunk then fills at once, without waiting for a preparation that
+        # takes longer than the chunk
+        self.prefetch = prefetch
         # keep_outputs: each result also holds the output slabs and station
         # samples of its last min(steps, ring) steps (the device ring)
         self.keep_outputs = keep_outputs
@@ -168,7 +173,7 @@ class Server:
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue: collections.deque = collections.deque()
         active: List[dict] = []
-        ready: List[dict] = []          # prepared during the last chunk, join at the next boundary
+        ready: List[dict] = []          # prepared, join at the first boundary with a free slot
         leaving: List[dict] = []        # left the batch at the last boundary, read during the next chunk
         results: List[Result] = []
         t0 = clock()
@@ -176,10 +181,10 @@ class Server:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 queue.append(pending.popleft())
-            for a in ready:
+            while ready and len(active) < self.max_slots:
+                a = ready.pop(0)
                 self.batch.add(a["solver"])
                 active.append(a)
-            ready = []
             n = min([self.chunk] + [a["left"] for a in active]) if active else 0
             t1 = clock()
             if n:
@@ -187,9 +192,10 @@ class Server:
             t2 = clock()
             self.host_s["submit"] += t2 - t1
             # admission: a free slot (members whose run ends with this chunk
-            # leave at its end) and the footprint in free HBM
+            # leave at its end) or room in the prefetch, and the footprint
+            # in free HBM
             staying = sum(1 for a in active if a["left"] > n)
-            while queue and staying + len(ready) < self.max_slots and self._fits(queue[0], 0):
+            while queue and staying + len(ready) < self.max_slots + self.prefetch and self._fits(queue[0], 0):
                 r = queue.popleft()
                 s = capi.Solver(r.case, **self.solver_args(r))
                 if r.bins is not None:

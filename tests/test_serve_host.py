@@ -4,6 +4,7 @@ requests."""
 import math
 
 import numpy as np
+import pytest
 
 from fd3d_tsn_amd import serve
 
@@ -59,6 +60,7 @@ class _FakeSolver:
     def __init__(self, log, case, **kw):
         self.log, self.case, self.done, self.closed, self.batch = log, case, 0, False, None
         self.bad = kw.get("newton_max") == 1
+        log.append(("made", case.rid))
 
     def set_source_bins(self, *a, **k):
         pass
@@ -136,15 +138,17 @@ def _fake_server(monkeypatch, log, **kw):
     return serve.Server(mode=FakeCapi.STAGED, reserve_bytes=0, **kw)
 
 
-def test_server_refills_slots_at_the_boundary_they_free(monkeypatch):
+@pytest.mark.parametrize("prefetch", [0, 2])
+def test_server_refills_slots_at_the_boundary_they_free(monkeypatch, prefetch):
     """Scheduling of the serving loop, on fakes: every request gets exactly
     its steps; no chunk runs with a free slot while requests wait (a slot
-    freed at a chunk's end is filled at that boundary); results are read
-    only after the member left the batch, and every solver is freed once."""
+    freed at a chunk's end is filled at that boundary); at most `prefetch`
+    prepared requests wait beyond the slots; results are read only after
+    the member left the batch, and every solver is freed once."""
     log = []
     steps = [20, 8, 35, 13, 8, 50, 27, 16, 9, 40]
     reqs = [serve.Request(q, 0.0, _Case(q, n)) for q, n in enumerate(steps)]
-    srv = _fake_server(monkeypatch, log, max_slots=3, chunk=8)
+    srv = _fake_server(monkeypatch, log, max_slots=3, chunk=8, prefetch=prefetch)
     got = srv.run(reqs)
     assert sorted(r.rid for r in got) == list(range(len(reqs)))
     for r in got:
@@ -157,6 +161,14 @@ def test_server_refills_slots_at_the_boundary_they_free(monkeypatch):
         waiting = len(reqs) - len(started)
         assert len(members) == 3 or waiting == 0, (members, waiting)
     assert srv.chunks == len(chunks)
+    # prepared solvers not yet in the batch, at every chunk
+    made = 0
+    for e in log:
+        if e[0] == "made":
+            made += 1
+        elif e[0] == "chunk":
+            closed = sum(1 for f in log[:log.index(e)] if f[0] == "close")
+            assert made - closed - len(e[2]) <= prefetch + 3, (made, closed, e)
 
 
 def test_server_failed_member_leaves_alone(monkeypatch):
