@@ -341,13 +341,12 @@ struct fd3d_batch {
   int fused_items = 8;   // least fused work items per CU for the fused stage
   int fixed_dev = -1;    // fd3d_batch_create_static: tables in the device's fixed scratch, else -1
   long long wait_ticks = 0;
-  // staged mode: head, rotated and tail graphs of a step (see queue_step_t);
+  // staged mode: head, rotated and tail graphs of a step (see capture_staged_t);
   // persistent mode: graph 0 only
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
-  hipStream_t side = nullptr;               // the fork's second stream
+  hipStream_t side = nullptr;               // the captured fork's second stream
   hipEvent_t fork = nullptr, join = nullptr;
-  bool capture_failed = false;   // the last capture failed: queue steps directly
   struct Ev { hipEvent_t a, b; int n; };
   std::vector<Ev> evs;
   std::vector<hipEvent_t> pool;
@@ -465,8 +464,7 @@ int capture_t(fd3d_batch *b) {
   return 0;
 }
 
-// Staged mode: one launch per stage covering every member, queued in three
-// kinds of part:
+// Staged mode: one launch per stage covering every member, captured as
 //   head    a step up to and including advance;
 //   rotated the previous step's fault-plane stages (traction .. nucleation)
 //           and then vel_fault on the side stream, beside vel_interior and
@@ -477,12 +475,10 @@ int capture_t(fd3d_batch *b) {
 // same old data and writes disjoint cells.  The fault-plane stages read the
 // finished step's new set and the fault arrays, which the velocity sweeps
 // of the next step neither write nor read (the kernels after advance take
-// the step it has counted).  A run of n steps is head, n-1 rotated, tail.
-// The parts are queued on the batch's own streams (the first run after a
-// membership change) or captured once on the capture streams into graphs
-// that later runs replay.
+// the step it has counted).  A run of n steps replays head, n-1 rotated,
+// tail.
 template <int TZ, bool FVW>
-int queue_step_t(fd3d_batch *b, int kind, const Lanes &L) {
+int capture_staged_t(fd3d_batch *b, int kind) {
   BatchArgs A{b->d_seg, b->d_dom, b->d_cnt, (int)b->segs.size(), b->total, b->wait_ticks};
   int launches = 0;
   auto stage = [&](int s, hipStream_t st) {
@@ -497,55 +493,42 @@ int queue_step_t(fd3d_batch *b, int kind, const Lanes &L) {
   };
   hipError_t e = hipSuccess;
   auto fork = [&]() {
-    if (e == hipSuccess) e = hipEventRecord(L.fork, L.main);
-    if (e == hipSuccess) e = hipStreamWaitEvent(L.side, L.fork, 0);
+    if (e == hipSuccess) e = hipEventRecord(b->fork, b->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->fork, 0);
   };
   auto join = [&]() {
-    if (e == hipSuccess) e = hipEventRecord(L.join, L.side);
-    if (e == hipSuccess) e = hipStreamWaitEvent(L.main, L.join, 0);
+    if (e == hipSuccess) e = hipEventRecord(b->join, b->side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join, 0);
   };
   static const int chain[] = {K_TRACTION, K_TRACTION_MIRROR, K_TABS, K_FRICTION, K_OUTPUTS, K_RATES, K_NUCLEATION};
   if (kind == 2) {
-    for (int s : chain) stage(s, L.main);
+    for (int s : chain) stage(s, b->stream);
   } else {
     fork();
     if (kind == 1)
-      for (int s : chain) stage(s, L.side);
-    stage(K_VEL_FAULT, L.side);
-    stage(K_VEL_INTERIOR, L.main);
-    stage(K_PML_VEL, L.main);
+      for (int s : chain) stage(s, b->side);
+    stage(K_VEL_FAULT, b->side);
+    stage(K_VEL_INTERIOR, b->stream);
+    stage(K_PML_VEL, b->stream);
     join();
-    stage(K_FUSED, L.main);
-    stage(K_FUVW_UV, L.main);
-    stage(K_FUVW_W, L.main);
+    stage(K_FUSED, b->stream);
+    stage(K_FUVW_UV, b->stream);
+    stage(K_FUVW_W, b->stream);
     fork();
-    stage(K_STR_FAULT, L.side);
-    stage(K_STR_TOP, L.main);
-    stage(K_STR_INTERIOR, L.main);
-    stage(K_PML_STR, L.main);
+    stage(K_STR_FAULT, b->side);
+    stage(K_STR_TOP, b->stream);
+    stage(K_STR_INTERIOR, b->stream);
+    stage(K_PML_STR, b->stream);
     join();
-    stage(K_FRES, L.main);
-    stage(K_ADVANCE, L.main);
+    stage(K_FRES, b->stream);
+    stage(K_ADVANCE, b->stream);
   }
-  if (e != hipSuccess) return bfail(b, FD3D_E_HIP, std::string("batch step fork/join: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return bfail(b, FD3D_E_HIP, std::string("batch graph fork/join: ") + hipGetErrorString(e));
   if (kind == 1) b->launches = launches;   // a step in a run: every stage once
   return 0;
 }
 static_assert(K_ADVANCE + 8 == K_COUNT && K_NUCLEATION == K_COUNT - 1,
-              "queue_step_t names every stage: advance, then the seven fault-plane stages");
-
-// One part (kind: head, rotated or tail) of a staged step on the lanes L,
-// for the batch's tile height and friction.
-int queue_step(fd3d_batch *b, int kind, const Lanes &L) {
-  switch (b->tz * 2 + (b->fvw ? 1 : 0)) {
-    case 8: return queue_step_t<4, false>(b, kind, L);
-    case 9: return queue_step_t<4, true>(b, kind, L);
-    case 16: return queue_step_t<8, false>(b, kind, L);
-    case 17: return queue_step_t<8, true>(b, kind, L);
-    case 32: return queue_step_t<16, false>(b, kind, L);
-    default: return queue_step_t<16, true>(b, kind, L);
-  }
-}
+              "capture_staged_t names every stage: advance, then the seven fault-plane stages");
 
 void drop_graphs(fd3d_batch *b) {
   for (int n = 0; n < 3; ++n) {
@@ -556,7 +539,6 @@ void drop_graphs(fd3d_batch *b) {
 
 int rebuild(fd3d_batch *b) {
   drop_graphs(b);
-  b->capture_failed = false;
   std::vector<BatchDomain> doms;
   b->fvw = false;
   bool fused = b->mode == FD3D_BATCH_STAGED;
@@ -618,70 +600,47 @@ int rebuild(fd3d_batch *b) {
     }
     if (rc) { b->dirty = true; return rc; }
   }
-  if (b->mode != FD3D_BATCH_PERSISTENT) return 0;   // staged: queued directly, captured after (capture_staged)
-  const hipError_t eb = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
-  if (eb != hipSuccess) {
-    b->dirty = true;
-    return bfail(b, FD3D_E_HIP, std::string("begin capture: ") + hipGetErrorString(eb));
-  }
-  switch (b->tz * 2 + (b->fvw ? 1 : 0)) {
-    case 8: rc = capture_t<4, false>(b); break;
-    case 9: rc = capture_t<4, true>(b); break;
-    case 16: rc = capture_t<8, false>(b); break;
-    case 17: rc = capture_t<8, true>(b); break;
-    case 32: rc = capture_t<16, false>(b); break;
-    default: rc = capture_t<16, true>(b); break;
-  }
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(b->stream, &g);
-  if (rc || e != hipSuccess) {
-    if (g) (void)hipGraphDestroy(g);
-    b->dirty = true;
-    return rc ? rc : bfail(b, FD3D_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-  }
-  b->graph[0] = g;
-  e = hipGraphInstantiate(&b->exec[0], g, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
-    drop_graphs(b);
-    b->dirty = true;
-    return bfail(b, FD3D_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-  }
-  return 0;
-}
-
-// Staged mode: the head, rotated and tail graphs, captured on the capture
-// lanes (which run nothing), so that the host captures them while the GPU
-// runs the steps just queued directly.  All three graphs or none; on
-// failure the batch keeps queueing its steps directly until the next
-// membership change.
-int capture_staged(fd3d_batch *b) {
-  Lanes L{};
-  std::unique_lock<std::mutex> lock;
-  if (fd3d_capture_lanes(L, lock) != 0) {
-    b->capture_failed = true;
-    return bfail(b, FD3D_E_HIP, "capture streams could not be made");
-  }
-  for (int kind = 0; kind < 3; ++kind) {
-    const hipError_t eb = hipStreamBeginCapture(L.main, hipStreamCaptureModeThreadLocal);
+  const int kinds = b->mode == FD3D_BATCH_PERSISTENT ? 1 : 3;
+  for (int kind = 0; kind < kinds; ++kind) {
+    const hipError_t eb = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (eb != hipSuccess) {
-      drop_graphs(b);
-      b->capture_failed = true;
+      drop_graphs(b);   // all graphs or none
+      b->dirty = true;
       return bfail(b, FD3D_E_HIP, std::string("begin capture: ") + hipGetErrorString(eb));
     }
-    const int rc = queue_step(b, kind, L);
+    const int key = b->tz * 2 + (b->fvw ? 1 : 0);
+    if (b->mode == FD3D_BATCH_PERSISTENT) {
+      switch (key) {
+        case 8: rc = capture_t<4, false>(b); break;
+        case 9: rc = capture_t<4, true>(b); break;
+        case 16: rc = capture_t<8, false>(b); break;
+        case 17: rc = capture_t<8, true>(b); break;
+        case 32: rc = capture_t<16, false>(b); break;
+        default: rc = capture_t<16, true>(b); break;
+      }
+    } else {
+      switch (key) {
+        case 8: rc = capture_staged_t<4, false>(b, kind); break;
+        case 9: rc = capture_staged_t<4, true>(b, kind); break;
+        case 16: rc = capture_staged_t<8, false>(b, kind); break;
+        case 17: rc = capture_staged_t<8, true>(b, kind); break;
+        case 32: rc = capture_staged_t<16, false>(b, kind); break;
+        default: rc = capture_staged_t<16, true>(b, kind); break;
+      }
+    }
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(L.main, &g);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc || e != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
       drop_graphs(b);
-      b->capture_failed = true;
+      b->dirty = true;
       return rc ? rc : bfail(b, FD3D_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
     }
     b->graph[kind] = g;
     e = hipGraphInstantiate(&b->exec[kind], g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
       drop_graphs(b);
-      b->capture_failed = true;
+      b->dirty = true;
       return bfail(b, FD3D_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
     }
   }
@@ -844,27 +803,16 @@ int fd3d_batch_step(fd3d_batch *b, int nsteps) {
   }
   hipEvent_t e0 = take_event(b), e1 = take_event(b);
   BCHK(b, hipEventRecord(e0, b->stream));
-  const bool graphs = b->exec[0] && b->exec[1] && b->exec[2];
   if (b->mode == FD3D_BATCH_PERSISTENT) {
     for (int s = 0; s < nsteps; ++s) BCHK(b, hipGraphLaunch(b->exec[0], b->stream));
-  } else if (graphs) {
+  } else {
     BCHK(b, hipGraphLaunch(b->exec[0], b->stream));
     for (int s = 1; s < nsteps; ++s) BCHK(b, hipGraphLaunch(b->exec[1], b->stream));
     BCHK(b, hipGraphLaunch(b->exec[2], b->stream));
-  } else {
-    // the first run after a membership change: queue the launches directly,
-    // so the GPU starts at once, and capture the graphs while it runs
-    const Lanes L{b->stream, b->side, b->fork, b->join};
-    for (int s = 0; s <= nsteps; ++s) {
-      const int rc = queue_step(b, s == 0 ? 0 : s == nsteps ? 2 : 1, L);
-      if (rc) return rc;
-    }
-    BCHK(b, hipGetLastError());
   }
   BCHK(b, hipEventRecord(e1, b->stream));
   b->evs.push_back({e0, e1, nsteps});
   for (fd3d_handle *h : b->mem) h->steps += nsteps;
-  if (b->mode != FD3D_BATCH_PERSISTENT && !graphs && !b->capture_failed) (void)capture_staged(b);
   return 0;
 }
 

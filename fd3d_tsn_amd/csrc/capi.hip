@@ -12,7 +12,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <mutex>
 #include <string>
 #include <vector>
 
@@ -368,7 +367,6 @@ void drop_graph(fd3d_handle *h) {
 
 void build_seq(fd3d_handle *h) {
   drop_graph(h);
-  h->capture_failed = false;
   const Problem &p = h->p;
   const bool fused = p.tune.mode & 4;
   h->seq.clear();
@@ -402,36 +400,6 @@ int fd3d_handle_check_ctl(fd3d_handle *h, hipStream_t s) {
     return fail(h, FD3D_E_NEWTON, "rate-and-state Newton solve hit the iteration cap at " +
                                       std::to_string(c.newton_capped) + " node-steps");
   if (c.nonfinite) return fail(h, FD3D_E_NONFINITE, "non-finite slip rate in the fault outputs");
-  return 0;
-}
-
-// Capture lanes per device (declared in handle.h).
-namespace {
-constexpr int kCaptureDevices = 64;
-std::mutex g_capture_mu;
-Lanes g_capture[kCaptureDevices];
-}  // namespace
-
-int fd3d_capture_lanes(Lanes &out, std::unique_lock<std::mutex> &lock) {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kCaptureDevices) return FD3D_E_HIP;
-  lock = std::unique_lock<std::mutex>(g_capture_mu);
-  Lanes &L = g_capture[dev];
-  if (!L.main) {
-    Lanes n{};
-    if (hipStreamCreateWithFlags(&n.main, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&n.side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&n.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&n.join, hipEventDisableTiming) != hipSuccess) {
-      if (n.fork) (void)hipEventDestroy(n.fork);
-      if (n.side) (void)hipStreamDestroy(n.side);
-      if (n.main) (void)hipStreamDestroy(n.main);
-      lock.unlock();
-      return FD3D_E_HIP;
-    }
-    L = n;
-  }
-  out = L;
   return 0;
 }
 
@@ -757,9 +725,9 @@ static int half_step_group(KernelId id, bool small) {
 // disjoint writes, neither reads what the other writes).
 constexpr long long kOverlapCells = 4LL << 20;
 
-// A step of n steps is queued in three kinds of part (kernel arguments are
-// fixed for the life of the handle: the wavefield set and the time come
-// from the device step counter, so the same parts serve every step):
+// The step captured into three graphs on h->stream (kernel arguments are
+// fixed for the life of the handle: the wavefield set and the time come from
+// the device step counter, so the graphs serve every step):
 //   head    the step up to and including advance;
 //   rotated the previous step's fault-plane kernels (traction .. nucleation)
 //           and then vel_fault on the side stream, beside this step's other
@@ -774,70 +742,58 @@ constexpr long long kOverlapCells = 4LL << 20;
 // of kernels of the same half-step group (half_step_group) the first goes
 // on the main stream and the rest on the side stream (fork and join by
 // events), so they run side by side.
-// Queues one part on the lanes L (launches, or a capture when L.main is
-// capturing); what names the failing launch.
-static hipError_t queue_part(fd3d_handle *h, int kind, const Lanes &L, std::string &what) {
+static int capture_graphs(fd3d_handle *h) {
   const bool small = (long long)h->cfg.nxt * h->cfg.nyt * h->cfg.nzt < kOverlapCells;
   const std::vector<KernelId> &seq = h->seq;
   const size_t nadv = (size_t)(std::find(seq.begin(), seq.end(), K_ADVANCE) - seq.begin());
+  if (nadv >= seq.size()) return fail(h, FD3D_E_STATE, "step sequence without advance");
   const std::vector<KernelId> pre(seq.begin(), seq.begin() + nadv + 1), chain(seq.begin() + nadv + 1, seq.end());
   size_t nlead = 0;   // leading velocity sweeps of the step
   while (nlead < pre.size() && (pre[nlead] == K_VEL_INTERIOR || pre[nlead] == K_VEL_FAULT || pre[nlead] == K_PML_VEL))
     ++nlead;
-  hipError_t e = hipSuccess;
-  auto on = [&](KernelId id, hipStream_t st) {
-    if (e == hipSuccess && (e = launch(id, h->p, st)) != hipSuccess) what = kKernelNames[id];
-  };
-  auto ev = [&](hipEvent_t evt, hipStream_t rec, hipStream_t wait) {
-    if (e == hipSuccess && (e = hipEventRecord(evt, rec)) == hipSuccess) e = hipStreamWaitEvent(wait, evt, 0);
-    if (e != hipSuccess && what.empty()) what = "fork/join";
-  };
-  // kernels q[from..] with the half-step groups side by side
-  auto grouped = [&](const std::vector<KernelId> &q, size_t from) {
-    for (size_t n = from; n < q.size() && e == hipSuccess;) {
-      size_t m = n + 1;
-      const int grp = half_step_group(q[n], small);
-      while (grp && m < q.size() && half_step_group(q[m], small) == grp) ++m;
-      if (m - n > 1) ev(L.fork, L.main, L.side);
-      on(q[n], L.main);
-      for (size_t r = n + 1; r < m; ++r) on(q[r], L.side);
-      if (m - n > 1) ev(L.join, L.side, L.main);
-      n = m;
-    }
-  };
-  if (kind == 0) {
-    grouped(pre, 0);
-  } else if (kind == 1) {
-    ev(L.fork, L.main, L.side);
-    for (KernelId id : chain) on(id, L.side);
-    on(K_VEL_FAULT, L.side);
-    for (size_t n = 0; n < nlead; ++n)
-      if (pre[n] != K_VEL_FAULT) on(pre[n], L.main);
-    ev(L.join, L.side, L.main);
-    grouped(pre, nlead);
-  } else {
-    for (KernelId id : chain) on(id, L.main);
-  }
-  return e;
-}
-
-// The three parts captured into graphs on the capture lanes, which run
-// nothing, so that the host captures them while the GPU runs the steps
-// just queued directly.  All three graphs or none.
-static int capture_graphs(fd3d_handle *h) {
-  Lanes L{};
-  std::unique_lock<std::mutex> lock;
-  if (fd3d_capture_lanes(L, lock) != 0) return fail(h, FD3D_E_HIP, "capture streams could not be made");
   for (int kind = 0; kind < 3; ++kind) {
-    const hipError_t eb = hipStreamBeginCapture(L.main, hipStreamCaptureModeThreadLocal);
+    const hipError_t eb = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
     if (eb != hipSuccess) {
-      drop_graph(h);
+      drop_graph(h);   // all three graphs or none
       return fail(h, FD3D_E_HIP, std::string("begin capture: ") + hipGetErrorString(eb));
     }
+    hipError_t e = hipSuccess;
     std::string what;
-    const hipError_t e = queue_part(h, kind, L, what);
+    auto on = [&](KernelId id, hipStream_t st) {
+      if (e == hipSuccess && (e = launch(id, h->p, st)) != hipSuccess) what = kKernelNames[id];
+    };
+    auto ev = [&](hipEvent_t evt, hipStream_t rec, hipStream_t wait) {
+      if (e == hipSuccess && (e = hipEventRecord(evt, rec)) == hipSuccess) e = hipStreamWaitEvent(wait, evt, 0);
+      if (e != hipSuccess && what.empty()) what = "fork/join";
+    };
+    // kernels q[from..] with the half-step groups side by side
+    auto grouped = [&](const std::vector<KernelId> &q, size_t from) {
+      for (size_t n = from; n < q.size() && e == hipSuccess;) {
+        size_t m = n + 1;
+        const int grp = half_step_group(q[n], small);
+        while (grp && m < q.size() && half_step_group(q[m], small) == grp) ++m;
+        if (m - n > 1) ev(h->fork, h->stream, h->side);
+        on(q[n], h->stream);
+        for (size_t r = n + 1; r < m; ++r) on(q[r], h->side);
+        if (m - n > 1) ev(h->join, h->side, h->stream);
+        n = m;
+      }
+    };
+    if (kind == 0) {
+      grouped(pre, 0);
+    } else if (kind == 1) {
+      ev(h->fork, h->stream, h->side);
+      for (KernelId id : chain) on(id, h->side);
+      on(K_VEL_FAULT, h->side);
+      for (size_t n = 0; n < nlead; ++n)
+        if (pre[n] != K_VEL_FAULT) on(pre[n], h->stream);
+      ev(h->join, h->side, h->stream);
+      grouped(pre, nlead);
+    } else {
+      for (KernelId id : chain) on(id, h->stream);
+    }
     hipGraph_t g = nullptr;
-    const hipError_t ee = hipStreamEndCapture(L.main, &g);
+    const hipError_t ee = hipStreamEndCapture(h->stream, &g);
     if (e != hipSuccess || ee != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
       drop_graph(h);
@@ -853,10 +809,8 @@ static int capture_graphs(fd3d_handle *h) {
   return 0;
 }
 
-// Queue n steps on h->stream: kernel by kernel with events when profiling;
-// else head, n-1 rotated and the tail part, replayed from the graphs, or,
-// before the graphs exist (the first run, or after the step sequence
-// changed), queued directly and the graphs captured while the GPU runs them.
+// Queue n steps on h->stream: kernel by kernel with events when profiling,
+// else head, n-1 rotated and the tail graph.
 static int run_steps(fd3d_handle *h, int n) {
   if (n <= 0) return 0;
   if (h->prof) {
@@ -867,24 +821,14 @@ static int run_steps(fd3d_handle *h, int n) {
       }
     return 0;
   }
-  if (h->gexec[0] && h->gexec[1] && h->gexec[2]) {
-    HIPCHK(h, hipGraphLaunch(h->gexec[0], h->stream));
-    for (int s = 1; s < n; ++s) HIPCHK(h, hipGraphLaunch(h->gexec[1], h->stream));
-    HIPCHK(h, hipGraphLaunch(h->gexec[2], h->stream));
-    return 0;
+  if (!h->gexec[0] || !h->gexec[1] || !h->gexec[2]) {
+    drop_graph(h);
+    int rc = capture_graphs(h);
+    if (rc) return rc;
   }
-  if (std::find(h->seq.begin(), h->seq.end(), K_ADVANCE) == h->seq.end())
-    return fail(h, FD3D_E_STATE, "step sequence without advance");
-  drop_graph(h);
-  const Lanes L{h->stream, h->side, h->fork, h->join};
-  for (int s = 0; s <= n; ++s) {
-    std::string what;
-    const hipError_t e = queue_part(h, s == 0 ? 0 : s == n ? 2 : 1, L, what);
-    if (e != hipSuccess) return fail(h, FD3D_E_HIP, "launch " + what + ": " + hipGetErrorString(e));
-  }
-  // the steps are queued; a failed capture (its message stays in the last
-  // error) leaves the handle queueing directly until the sequence changes
-  if (!h->capture_failed && capture_graphs(h) != 0) h->capture_failed = true;
+  HIPCHK(h, hipGraphLaunch(h->gexec[0], h->stream));
+  for (int s = 1; s < n; ++s) HIPCHK(h, hipGraphLaunch(h->gexec[1], h->stream));
+  HIPCHK(h, hipGraphLaunch(h->gexec[2], h->stream));
   return 0;
 }
 

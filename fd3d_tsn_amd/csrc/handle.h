@@ -3,7 +3,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include <mutex>
 #include <string>
 #include <vector>
 
@@ -11,18 +10,6 @@
 #include "fd3d_dev.h"
 
 using namespace fd3d;
-
-// The two streams and the fork/join events a step is queued on.
-struct Lanes {
-  hipStream_t main, side;
-  hipEvent_t fork, join;
-};
-
-// The lanes step graphs are captured on: one set per device, shared by every
-// problem and batch of the process, since nothing runs on them.  Made at
-// first use on the current device; a capture holds the returned lock.
-// Returns 0 or FD3D_E_HIP (capi.hip).
-int fd3d_capture_lanes(Lanes &out, std::unique_lock<std::mutex> &lock);
 
 struct fd3d_handle {
   fd3d_config cfg{};
@@ -47,19 +34,18 @@ struct fd3d_handle {
   double kms[K_COUNT] = {0};
   long kcount[K_COUNT] = {0};
   // step sequence, and three graphs of it (replayed by fd3d_step when
-  // profiling is off; captured after the first run of a sequence, which is
-  // queued directly): a run of n steps is the head part (a step up to
-  // advance), n-1 rotated parts (the previous step's fault-plane kernels
-  // beside this step's velocity sweeps, then this step up to advance) and
-  // the tail part (the last step's fault-plane kernels)
+  // profiling is off; rebuilt when the sequence changes): a run of n steps
+  // is the head graph (a step up to advance), n-1 rotated graphs (the
+  // previous step's fault-plane kernels beside this step's velocity
+  // sweeps, then this step up to advance) and the tail graph (the last
+  // step's fault-plane kernels)
   std::vector<KernelId> seq;
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
-  // second stream and fork/join events of a step: the kernels of a
-  // half-step that write disjoint cells run side by side
+  // second stream and fork/join events of the captured step: the kernels of
+  // a half-step that write disjoint cells run side by side
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
-  bool capture_failed = false;   // the last capture failed: queue steps directly
   // the batch this problem is advanced in (fd3d_batch_add), or null
   fd3d_batch *batch = nullptr;
   // overlapped output stream (fd3d_step_begin / fd3d_step_wait): two pinned

@@ -240,11 +240,9 @@ void fd3d_destroy(fd3d_handle *h);
  * problems of different sizes, friction laws and free-surface settings
  * together (FD3D_BATCH_STAGED): each stage of a step is one launch covering
  * every problem, whose workgroups take (problem, stage, block) items from a
- * table; the launches of a step are captured in HIP graphs and replayed
- * (the first run after a membership change is queued directly while the
- * graphs are captured), with each step's fault-plane stages beside the
- * next step's velocity stages.  Results are bit-identical to fd3d_step on
- * each problem alone.
+ * table; the launches of a step are captured in HIP graphs and replayed,
+ * with each step's fault-plane stages beside the next step's velocity
+ * stages.  Results are bit-identical to fd3d_step on each problem alone.
  * ------------------------------------------------------------------------ */
 typedef struct fd3d_batch fd3d_batch;
 
@@ -297,8 +295,8 @@ int fd3d_batch_sync(fd3d_batch *b);
  * own) when the batch's fused work items reach
  * min_items_per_cu per compute unit (default 8; 0: whenever a member asks);
  * otherwise every member runs the separate sweeps, such members with ny_str
- * rows for both.  Results are bit-identical either way.  The next step
- * re-plans the batch and captures its graphs again. */
+ * rows for both.  Results are bit-identical either way.  Marks the step
+ * graph for re-capture. */
 int fd3d_batch_set_fused_items(fd3d_batch *b, int min_items_per_cu);
 
 /* out4: resident workgroups of the persistent launch, work items per step,
