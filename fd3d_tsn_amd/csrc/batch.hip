@@ -1,5 +1,5 @@
 // Multi-domain batch engine: many resident problems of different sizes
-// advanced together, one time step per replay of a captured HIP graph.
+// advanced together, their steps replayed from captured HIP graphs.
 //
 // Work decomposition.  Every stage of one problem's step (the kernel
 // sequence of capi.hip) is cut into work items of one 64 x TZ workgroup:
@@ -26,10 +26,12 @@
 // Staged mode (FD3D_BATCH_STAGED): one launch per stage covering every
 // problem; kernel boundaries order the stages.
 //
-// Either way the launches of one step are captured once into a HIP graph
-// (re-captured when the membership changes) and the graph is replayed per
-// step.  The graph holds kernel nodes only: the per-step reset of the
-// queue and counters is a kernel, not a memset.
+// Either way the launches are captured into HIP graphs, re-captured when
+// the membership changes: the persistent form's step is one graph; the
+// staged form's run of n steps is a head, n-1 rotated and a tail graph, so
+// that each step's fault-plane stages run beside the next step's velocity
+// stages (capture_staged_t).  The graphs hold kernel nodes only: the
+// per-step reset of the queue and counters is a kernel, not a memset.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
