@@ -240,10 +240,12 @@ class Server:
         """Read the result of a member that has left the batch and free it;
         the request is finished once its arrays are on the host."""
         s, r, err = a["solver"], a["req"], a["error"]
-        fin = s.final() if self.keep_final and not err else {}
-        bins = s.source_bins() if r.bins is not None and not err else {}
-        outs = s.read_outputs(min(r.steps, s.ring_steps)) if self.keep_outputs and not err else {}
-        s.close()
+        try:
+            fin = s.final() if self.keep_final and not err else {}
+            bins = s.source_bins() if r.bins is not None and not err else {}
+            outs = s.read_outputs(min(r.steps, s.ring_steps)) if self.keep_outputs and not err else {}
+        finally:
+            s.close()
         return Result(r.rid, r.arrival, a["admitted"], now(), r.steps, r.case.cells, fin, bins, outs, err)
 
     def _sync(self) -> Dict[int, str]:
